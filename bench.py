@@ -1,0 +1,191 @@
+"""Benchmark: the 4K RAW12 Malvar ISP pipeline on one MI355X.
+
+Workload (one "step" = one frame): 3840x2160 GRBG MIPI RAW12 (packed, 5760 B rows)
+seeded noisy scene, black level 240, 17x13 lens shading, gray-world white
+balance estimated on the frame, Malvar-He-Cutler demosaic, unsharp mask 0.5,
+fixed CCM, sRGB tone table, RGBA8 output, 4x256 histograms, statistics and an
+auto-exposure suggestion. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: Gpix/s (value), HBM GB/s at the compulsory 5.5 B/px, per-
+frame p50/p99 GPU time, a bit-exact check of the last frame against the CPU
+oracle, the dominant kernel's roofline (HIP events on the launch stream), and
+the reference CPU backend timed on this host's cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "holoscan-zozo-modules_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+RAW_BYTES_PER_PX = 1.5  # RAW12
+OUT_BYTES_PER_PX = 4.0  # RGBA8
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-frames", type=int, default=6, help="frames of the single-thread CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    return ap.parse_args()
+
+
+def percentile(values, q):
+    s = sorted(values)
+    if not s:
+        return None
+    k = min(len(s) - 1, max(0, int(round(q / 100.0 * (len(s) - 1)))))
+    return s[k]
+
+
+def cpu_baseline(frame, params, out_format, frames):
+    """The reference CPU backend (compiled from its own sources) when present, else the
+    oracle; 1 thread, one warm-up call then the mean of `frames` calls
+    (modules/isp/tests/benchmark.cpp:147-165). Also the oracle on all cores."""
+    import oracle_lib
+    pixels = frame.width * frame.height
+    out = {}
+    if oracle_lib.ref_available():
+        pipe, kind = oracle_lib.RefPipeline(params), "reference"
+        run = lambda: pipe.process(frame, out_format, True, (10_000_000, 2.0))  # noqa: E731
+    else:
+        pipe, kind = oracle_lib.OraclePipeline(params), "port"
+        run = lambda: pipe.process(frame, out_format, True, (10_000_000, 2.0), threads=1)  # noqa: E731
+    run()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        run()
+    dt = (time.perf_counter() - t0) / frames
+    out["single"] = dict(value=pixels / dt / 1e9, unit="Gpix/s", cores=1, kind=kind,
+                         sample=f"{frames} frames of the 3840x2160 RAW12 workload after 1 warm-up, "
+                                f"{dt * 1e3:.1f} ms/frame")
+    threads = min(16, os.cpu_count() or 1)
+    orc = oracle_lib.OraclePipeline(params)
+    orc.process(frame, out_format, True, (10_000_000, 2.0), threads=threads)
+    n = max(2, frames)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        orc.process(frame, out_format, True, (10_000_000, 2.0), threads=threads)
+    dt2 = (time.perf_counter() - t0) / n
+    out["all_cores"] = dict(value=pixels / dt2 / 1e9, unit="Gpix/s", cores=threads, kind="port",
+                            ms_per_frame=dt2 * 1e3)
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import holoscan_isp as isp
+    import cases
+    import gpu_support as gs
+    from oracle_lib import OraclePipeline
+
+    torch.cuda.set_device(0)
+    scene, params, out_format = cases.config2()
+    frame = isp.render_scene(scene)
+    W, H = frame.width, frame.height
+    pixels = W * H
+    d_raw = torch.from_numpy(frame.data).to("cuda")
+    pitch = W * 4
+    out = torch.empty(pitch * H, dtype=torch.uint8, device="cuda")
+    hist = torch.empty(4 * 256, dtype=torch.int32, device="cuda")
+    stats = torch.empty(12, dtype=torch.float32, device="cuda")
+    pipe = isp.IspPipeline(params)
+    view = frame.view(d_raw)
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    exposure = (10_000_000, 2.0)
+
+    def step():
+        pipe.process(view, out, pitch, out_format, hist, stats, exposure, sh)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # Timed region: K frames back to back.
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        step()
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(end)
+    elapsed = max(wall, gpu_ms / 1e3)
+    gpix = args.steps * pixels / elapsed / 1e9
+    gbs = args.steps * pixels * (RAW_BYTES_PER_PX + OUT_BYTES_PER_PX) / elapsed / 1e9
+
+    # Kernel timing pass: HIP events around each launch on the launch stream.
+    pipe.timing_begin()
+    frame_ev = []
+    for _ in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        step()
+        e1.record(stream)
+        frame_ev.append((e0, e1))
+    torch.cuda.synchronize()
+    awb_ms, main_ms, n = pipe.timing_end()
+    per_frame = [a.elapsed_time(b) for a, b in frame_ev]
+    main_avg_s = main_ms / n / 1e3
+    algo_bytes = pixels * (RAW_BYTES_PER_PX + OUT_BYTES_PER_PX)
+    achieved = algo_bytes / main_avg_s / 1e9
+
+    # Bit-exact check: a fresh pipeline's first frame against the CPU oracle's.
+    threads = min(16, os.cpu_count() or 1)
+    g1 = gs.run(isp.IspPipeline(params), frame, out_format, True, exposure, d_raw=d_raw)
+    h1 = OraclePipeline(params).process(frame, out_format, True, exposure, threads=threads)
+    bit_exact = bool(gs.mismatches(g1, h1) == 0 and np.array_equal(g1.histogram, h1.histogram)
+                     and gs.same_stats(g1, h1))
+
+    result = {
+        "metric": "Gpix/s + HBM GB/s (bit-exact); p99 frame latency, batched vs per-frame",
+        "value": round(gpix, 3),
+        "unit": "Gpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "3840x2160 GRBG MIPI RAW12 seeded noisy scene, BL 240, 17x13 LSC, gray-world WB, "
+                        "Malvar, sharpen 0.5, CCM, sRGB, RGBA8 + 4x256 histograms + stats/AE, per-frame launches",
+            "width": W, "height": H, "frames": args.steps,
+        },
+        "hbm_gbs": round(gbs, 2),
+        "compulsory_bytes_per_px": RAW_BYTES_PER_PX + OUT_BYTES_PER_PX,
+        "frame_ms_p50": round(percentile(per_frame, 50), 5),
+        "frame_ms_p99": round(percentile(per_frame, 99), 5),
+        "kernel_ms": {"awb_avg": round(awb_ms / n, 5), "main_avg": round(main_ms / n, 5)},
+        "bit_exact": bit_exact,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "main_kernel<malvar,sharpen>", "algorithmic_bytes_per_launch": algo_bytes},
+    }
+    if not args.no_cpu:
+        base = cpu_baseline(frame, params, out_format, args.cpu_frames)
+        result["cpu_baseline"] = base["single"]
+        result["cpu_all_cores"] = base["all_cores"]
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

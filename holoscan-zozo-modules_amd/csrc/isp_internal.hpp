@@ -1,0 +1,118 @@
+// isp_internal.hpp - the library's internal C++ types, shared by the host code
+// and the HIP kernels.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/isp_c.h"
+#include "isp_ae.hpp"
+
+namespace isp_amd {
+
+inline constexpr int kAwbBins = 1024;      // white-patch histogram bins (isp_math.hpp:32)
+inline constexpr int kMaxHistBins = 1024;  // output histograms: 256 or the 1024-bin extension
+
+// A validated IspParams with owned tables (the reference copies IspParams, isp_pipeline.cpp:118).
+struct Params {
+  isp_params c{};
+  std::vector<float> lsc;  // lens shading gains, when c.has_lens_shading
+  std::vector<float> lut;  // the user's table, for ISP_TONE_LUT
+  int bins() const { return c.histogram_bins == ISP_HISTOGRAM_BINS_WIDE ? ISP_HISTOGRAM_BINS_WIDE : ISP_HISTOGRAM_BINS; }
+};
+
+struct invalid_argument : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct runtime_failure : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Host-side helpers (isp_params.cpp).
+Params copy_params(const isp_params* params);  // validates
+void validate_params(const isp_params& params);
+void validate_raw(const isp_raw_view& raw, const isp_params& params);
+void validate_out(const isp_raw_view& raw, const isp_rgb_view& out, const isp_params& params);
+std::vector<float> tone_lut(const isp_params& params);
+std::size_t raw_row_bytes(int format, int width);
+void set_error(const std::string& message);
+void clear_error();
+
+// Everything one frame's kernels read, passed by value (FrameParams,
+// isp_math.hpp:35-60, plus the outputs and the statistics switches).
+struct FrameDesc {
+  int width;
+  int height;
+  const std::uint8_t* raw;
+  long long raw_pitch;
+  int raw_format;
+  int cfa;  // 4 x 2 bits: channel at quad position (y & 1) * 2 + (x & 1)
+  float black[4];
+  float inv_range[4];
+  int white;
+  float wb[3];
+  const float* lsc;  // device grid, or null
+  int lsc_width;
+  int lsc_height;
+  int lsc_channels;
+  float lsc_sx;
+  float lsc_sy;
+  float ccm[9];
+  const float* lut;  // device table
+  int lut_size;
+  float sharpen;
+  std::uint8_t* out;
+  long long out_pitch;
+  int out_format;
+  std::uint32_t* histogram;  // user output or null
+  float* stats;              // user output or null
+  int bins;
+  // The frame's capture settings and the auto-exposure bounds, for finish_stats.
+  long long exposure_ns;
+  float exposure_gain;
+  AeParams ae;
+  // White balance: mode, and the finish's settings.
+  int wb_mode;
+  float percentile;
+  float smoothing;
+};
+
+// Per-pipeline device scratch, zero between frames (each pass's last block clears
+// what it consumed).
+struct AwbScratch {
+  unsigned long long sum[3];
+  unsigned long long quads;
+  unsigned int hist[3][kAwbBins];
+  unsigned int done;  // blocks finished
+  unsigned int pad;
+};
+struct WbState {
+  float gains[3];
+  int valid;
+};
+struct MainScratch {
+  unsigned long long sum[4];  // R, G, B fixed sums; saturated samples
+  unsigned int hist[4][kMaxHistBins];
+  unsigned int done;
+  unsigned int pad;
+};
+struct DeviceScratch {
+  AwbScratch awb;
+  WbState wb;
+  MainScratch main;
+};
+
+// Kernel launchers (isp_kernels.hip). `stream` is a hipStream_t.
+struct LaunchConfig {
+  int cu_count;
+  int tile_w;
+  int tile_h;
+};
+void launch_awb(const FrameDesc& f, DeviceScratch* scratch, const LaunchConfig& cfg, void* stream);
+void launch_main(const FrameDesc& f, DeviceScratch* scratch, bool malvar, bool sharpen,
+                 const LaunchConfig& cfg, void* stream);
+
+}  // namespace isp_amd

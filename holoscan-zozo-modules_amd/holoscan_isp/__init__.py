@@ -1,0 +1,223 @@
+"""holoscan_isp: Python access to the MI355X ISP through its C ABI (include/isp_c.h).
+
+`IspPipeline` mirrors holoscan::isp::IspPipeline (reference
+modules/isp/include/holoscan/isp/isp_pipeline.hpp:39-74): construct with
+IspParams, then process() frames whose buffers live in device memory (torch
+tensors on the ROCm device). Errors raise ValueError (std::invalid_argument)
+or RuntimeError (std::runtime_error) with the library's message.
+
+The library (libholoscan_isp_amd.so, built in-tree by `make -C
+holoscan-zozo-modules_amd`) must be present: there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import *  # noqa: F401,F403  (enums and constants)
+from .params import (AutoExposureParams, IspParams, LensShading, output_channel_bytes,  # noqa: F401
+                     output_channels, raw_row_bytes)
+
+PACKAGE_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBRARY = os.path.join(PACKAGE_DIR, "libholoscan_isp_amd.so")
+
+_lib = None
+
+
+def lib():
+    """The bound C library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise RuntimeError(f"{LIBRARY} is missing: build it with `make -C {PACKAGE_DIR}` "
+                               "(or __graft_entry__.build())")
+        _lib = _abi.bind(C.CDLL(LIBRARY))
+    return _lib
+
+
+def last_error():
+    return lib().isp_last_error().decode()
+
+
+def check(status, what="isp"):
+    if status == _abi.OK:
+        return
+    message = f"{what}: {last_error()}"
+    if status == _abi.INVALID_ARGUMENT:
+        raise ValueError(message)
+    if status == _abi.OUT_OF_MEMORY:
+        raise MemoryError(message)
+    raise RuntimeError(message)
+
+
+def default_params():
+    """IspParams as isp_params_init() fills them."""
+    p = _abi.Params()
+    lib().isp_params_init(C.byref(p))
+    return IspParams.from_c(p)
+
+
+def tone_lut(params):
+    c, keep = params.to_c()
+    size = C.c_int32()
+    check(lib().isp_tone_lut(C.byref(c), None, 0, C.byref(size)), "tone_lut")
+    out = np.zeros(size.value, dtype=np.float32)
+    check(lib().isp_tone_lut(C.byref(c), out.ctypes.data_as(C.POINTER(C.c_float)), size.value,
+                             C.byref(size)), "tone_lut")
+    return out
+
+
+def validate_params(params):
+    c, keep = params.to_c()
+    check(lib().isp_validate_params(C.byref(c)), "validate_params")
+
+
+def _ptr(x):
+    """Device or host address of a torch tensor / numpy array / int."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    return x.ctypes.data
+
+
+class RawFrame:
+    """A raw mosaic in host memory: `data` uint8 [height * pitch] and its view fields."""
+
+    def __init__(self, data, width, height, pitch, fmt, bit_depth, pattern):
+        self.data = np.ascontiguousarray(data, dtype=np.uint8)
+        self.width, self.height, self.pitch = int(width), int(height), int(pitch)
+        self.format, self.bit_depth, self.pattern = int(fmt), int(bit_depth), int(pattern)
+
+    def view(self, data=None):
+        return _abi.RawView(_ptr(data) if data is not None else self.data.ctypes.data,
+                            self.width, self.height, self.pitch, self.format, self.bit_depth,
+                            self.pattern)
+
+    @property
+    def pixels(self):
+        return self.width * self.height
+
+
+def synthetic_config(**kw):
+    """SyntheticRawConfig (synthetic_raw.hpp:40-59) with its defaults, fields overridden by kw."""
+    c = _abi.SyntheticConfig()
+    lib().isp_synthetic_config_init(C.byref(c))
+    for key, value in kw.items():
+        if isinstance(value, (list, tuple)):
+            arr = getattr(c, key)
+            for i, v in enumerate(value):
+                arr[i] = v
+        else:
+            setattr(c, key, value)
+    return c
+
+
+def render_synthetic(config, pitch=None):
+    """render_synthetic_raw into host memory; rows padded to even bytes as the reference's tests do."""
+    row = lib().isp_raw_row_bytes(config.format, config.width)
+    if pitch is None:
+        pitch = (row + 1) & ~1
+    data = np.zeros(pitch * config.height, dtype=np.uint8)
+    check(lib().isp_synthetic_render(C.byref(config), data.ctypes.data, pitch), "synthetic_render")
+    return RawFrame(data, config.width, config.height, pitch, config.format, config.bit_depth,
+                    config.pattern)
+
+
+def synthetic_params(config, grid=17):
+    """synthetic_isp_params(config, grid) as IspParams."""
+    p = _abi.Params()
+    gains = (C.c_float * (grid * grid))()
+    check(lib().isp_synthetic_params(C.byref(config), grid, C.byref(p), gains), "synthetic_params")
+    return IspParams.from_c(p)
+
+
+def synthetic_truth(config):
+    """The truth image, float32 [H, W, 3]."""
+    out = np.zeros((config.height, config.width, 3), dtype=np.float32)
+    check(lib().isp_synthetic_truth_image(C.byref(config), out.ctypes.data_as(C.POINTER(C.c_float))),
+          "synthetic_truth")
+    return out
+
+
+def scene_config(**kw):
+    c = _abi.SceneConfig()
+    lib().isp_scene_config_init(C.byref(c))
+    for key, value in kw.items():
+        if isinstance(value, (list, tuple)):
+            arr = getattr(c, key)
+            for i, v in enumerate(value):
+                arr[i] = v
+        else:
+            setattr(c, key, value)
+    return c
+
+
+def render_scene(config, pitch=None):
+    row = lib().isp_raw_row_bytes(config.format, config.width)
+    if pitch is None:
+        pitch = (row + 15) & ~15 if config.format != _abi.RAW_U16 else row
+    data = np.zeros(pitch * config.height, dtype=np.uint8)
+    check(lib().isp_scene_render(C.byref(config), data.ctypes.data, pitch), "scene_render")
+    return RawFrame(data, config.width, config.height, pitch, config.format, config.bit_depth,
+                    config.pattern)
+
+
+def scene_lens_shading(config, gw=17, gh=13):
+    gains = np.zeros(gw * gh, dtype=np.float32)
+    check(lib().isp_scene_lens_shading(C.byref(config), gw, gh,
+                                       gains.ctypes.data_as(C.POINTER(C.c_float))), "scene_lens_shading")
+    return LensShading(gw, gh, 1, gains)
+
+
+class IspPipeline:
+    """IspPipeline (isp_pipeline.hpp:39-74) on the GPU; buffers are device addresses."""
+
+    def __init__(self, params, backend=_abi.BACKEND_GPU):
+        self.params = params
+        self._c, self._keep = params.to_c()
+        h = C.c_void_p()
+        check(lib().isp_create(C.byref(self._c), backend, C.byref(h)), "IspPipeline")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().isp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def backend(self):
+        return lib().isp_backend(self._h)
+
+    def location(self):
+        return lib().isp_location(self._h)
+
+    def reset(self):
+        check(lib().isp_reset(self._h), "reset")
+
+    def timing_begin(self):
+        check(lib().isp_timing_begin(self._h), "timing_begin")
+
+    def timing_end(self):
+        """(awb_ms, main_ms, frames): summed kernel times since timing_begin()."""
+        a, m, n = C.c_double(), C.c_double(), C.c_int32()
+        check(lib().isp_timing_end(self._h, C.byref(a), C.byref(m), C.byref(n)), "timing_end")
+        return a.value, m.value, n.value
+
+    def process(self, raw_view, out, out_pitch, out_format, histogram=None, stats=None,
+                exposure=(0, 0.0), stream=None):
+        """Queues one frame: raw_view an _abi.RawView with a device address; out, histogram and
+        stats device tensors/addresses; stream a hipStream_t as int (None: the null stream)."""
+        rgb = _abi.RgbView(_ptr(out), out_pitch, out_format)
+        so = _abi.StatsOut(_ptr(histogram), _ptr(stats))
+        e = _abi.FrameExposure(int(exposure[0]), float(exposure[1]))
+        check(lib().isp_process(self._h, C.byref(raw_view), C.byref(rgb), C.byref(so), C.byref(e),
+                                stream), "process")

@@ -1,0 +1,148 @@
+"""ctypes mirror of include/isp_c.h and include/isp_synthetic.h.
+
+The field order and types follow the C headers exactly; the names follow the
+reference's holoscan::isp types (modules/isp/include/holoscan/isp/isp_params.hpp).
+"""
+import ctypes as C
+
+# enums (isp_params.hpp:21-61)
+RGGB, BGGR, GRBG, GBRG = 0, 1, 2, 3
+RAW_U8, RAW_U16, RAW_RAW10, RAW_RAW12 = 0, 1, 2, 3
+BILINEAR, MALVAR = 0, 1
+WB_MANUAL, WB_GRAY_WORLD, WB_WHITE_PATCH = 0, 1, 2
+TONE_SRGB, TONE_GAMMA, TONE_LINEAR, TONE_LUT = 0, 1, 2, 3
+OUT_RGB8, OUT_RGBA8, OUT_RGB16, OUT_RGB32F = 0, 1, 2, 3
+BACKEND_GPU, BACKEND_CPU, BACKEND_AUTO = 0, 1, 2
+LOCATION_HOST, LOCATION_DEVICE = 0, 1
+SCENE_CHART, SCENE_GRAY = 0, 1
+
+TONE_LUT_SIZE = 4097
+HISTOGRAM_BINS = 256
+HISTOGRAM_BINS_WIDE = 1024
+HISTOGRAM_CHANNELS = 4
+STATS_SIZE = 12
+(STAT_MEAN_R, STAT_MEAN_G, STAT_MEAN_B, STAT_MEAN_LUMA, STAT_AE_ERROR_EV, STAT_EXPOSURE_SCALE,
+ STAT_SUGGESTED_EXPOSURE_NS, STAT_SUGGESTED_GAIN, STAT_WB_GAIN_R, STAT_WB_GAIN_G, STAT_WB_GAIN_B,
+ STAT_SATURATED_FRACTION) = range(12)
+
+OK, INVALID_ARGUMENT, FAILURE, OUT_OF_MEMORY, NOT_READY = 0, 1, 2, 3, 4
+
+
+class LensShading(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32),
+                ("gains", C.POINTER(C.c_float)), ("gains_count", C.c_int64)]
+
+
+class AutoExposure(C.Structure):
+    _fields_ = [("target", C.c_float), ("damping", C.c_float),
+                ("min_exposure_ns", C.c_int64), ("max_exposure_ns", C.c_int64),
+                ("min_gain", C.c_float), ("max_gain", C.c_float)]
+
+
+class Params(C.Structure):
+    _fields_ = [("black_level", C.c_float),
+                ("has_channel_black_levels", C.c_uint8), ("channel_black_levels", C.c_float * 4),
+                ("white_level", C.c_float),
+                ("has_lens_shading", C.c_uint8), ("lens_shading", LensShading),
+                ("white_balance", C.c_uint8), ("wb_gains", C.c_float * 3),
+                ("awb_smoothing", C.c_float), ("white_patch_percentile", C.c_float),
+                ("demosaic", C.c_uint8), ("ccm", C.c_float * 9),
+                ("tone_curve", C.c_uint8), ("gamma", C.c_float),
+                ("lut", C.POINTER(C.c_float)), ("lut_size", C.c_int32),
+                ("sharpen", C.c_float), ("auto_exposure", AutoExposure),
+                ("histogram_bins", C.c_int32)]
+
+
+class RawView(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32),
+                ("pitch", C.c_size_t), ("format", C.c_uint8), ("bit_depth", C.c_int32),
+                ("pattern", C.c_uint8)]
+
+
+class RgbView(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("pitch", C.c_size_t), ("format", C.c_uint8)]
+
+
+class StatsOut(C.Structure):
+    _fields_ = [("histogram", C.c_void_p), ("stats", C.c_void_p)]
+
+
+class FrameExposure(C.Structure):
+    _fields_ = [("exposure_ns", C.c_int64), ("gain", C.c_float)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("stream_index", C.c_int32), ("raw", RawView), ("out", RgbView),
+                ("stats", StatsOut), ("exposure", FrameExposure)]
+
+
+class SyntheticConfig(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("pattern", C.c_uint8),
+                ("format", C.c_uint8), ("bit_depth", C.c_int32),
+                ("black_levels", C.c_float * 4), ("wb_gains", C.c_float * 3),
+                ("ccm", C.c_float * 9), ("vignetting", C.c_float), ("kind", C.c_uint8),
+                ("exposure", C.c_float), ("frame", C.c_uint32)]
+
+
+class SceneConfig(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("pattern", C.c_uint8),
+                ("format", C.c_uint8), ("bit_depth", C.c_int32), ("black_level", C.c_float),
+                ("seed", C.c_uint64), ("rectangles", C.c_int32),
+                ("corner_falloff", C.c_float), ("noise_gain", C.c_float),
+                ("read_sigma", C.c_float), ("illumination", C.c_float * 3)]
+
+
+P = C.POINTER
+_void = C.c_void_p
+
+# (name, restype, argtypes) of every symbol the headers declare.
+PROTOTYPES = [
+    ("isp_params_init", None, [P(Params)]),
+    ("isp_validate_params", C.c_int, [P(Params)]),
+    ("isp_validate_raw", C.c_int, [P(RawView), P(Params)]),
+    ("isp_raw_row_bytes", C.c_size_t, [C.c_uint8, C.c_int32]),
+    ("isp_tone_lut", C.c_int, [P(Params), P(C.c_float), C.c_int32, P(C.c_int32)]),
+    ("isp_output_channels", C.c_int32, [C.c_uint8]),
+    ("isp_output_channel_bytes", C.c_int32, [C.c_uint8]),
+    ("isp_backend_name", C.c_char_p, [C.c_uint8]),
+    ("isp_parse_backend", C.c_int, [C.c_char_p, P(C.c_uint8)]),
+    ("isp_demosaic_name", C.c_char_p, [C.c_uint8]),
+    ("isp_parse_demosaic", C.c_int, [C.c_char_p, P(C.c_uint8)]),
+    ("isp_pattern_name", C.c_char_p, [C.c_uint8]),
+    ("isp_parse_pattern", C.c_int, [C.c_char_p, P(C.c_uint8)]),
+    ("isp_white_balance_name", C.c_char_p, [C.c_uint8]),
+    ("isp_parse_white_balance", C.c_int, [C.c_char_p, P(C.c_uint8)]),
+    ("isp_probe_platform", C.c_int, []),
+    ("isp_resolve_auto", C.c_uint8, [C.c_int]),
+    ("isp_resolve_backend", C.c_uint8, [C.c_uint8]),
+    ("isp_last_error", C.c_char_p, []),
+    ("isp_epilogue_log2", C.c_double, [C.c_double]),
+    ("isp_epilogue_exp2", C.c_double, [C.c_double]),
+    ("isp_epilogue_finish_stats", None, [P(C.c_uint64), C.c_int64, P(C.c_float), P(AutoExposure),
+                                         P(FrameExposure), P(C.c_float)]),
+    ("isp_create", C.c_int, [P(Params), C.c_uint8, P(_void)]),
+    ("isp_destroy", C.c_int, [_void]),
+    ("isp_backend", C.c_uint8, [_void]),
+    ("isp_location", C.c_uint8, [_void]),
+    ("isp_reset", C.c_int, [_void]),
+    ("isp_process", C.c_int, [_void, P(RawView), P(RgbView), P(StatsOut), P(FrameExposure), _void]),
+    ("isp_timing_begin", C.c_int, [_void]),
+    ("isp_timing_end", C.c_int, [_void, P(C.c_double), P(C.c_double), P(C.c_int32)]),
+    ("isp_synthetic_config_init", None, [P(SyntheticConfig)]),
+    ("isp_synthetic_render", C.c_int, [P(SyntheticConfig), _void, C.c_size_t]),
+    ("isp_synthetic_params", C.c_int, [P(SyntheticConfig), C.c_int32, P(Params), P(C.c_float)]),
+    ("isp_synthetic_truth", C.c_int, [P(SyntheticConfig), C.c_int32, C.c_int32, P(C.c_float)]),
+    ("isp_synthetic_truth_image", C.c_int, [P(SyntheticConfig), P(C.c_float)]),
+    ("isp_scene_config_init", None, [P(SceneConfig)]),
+    ("isp_scene_render", C.c_int, [P(SceneConfig), _void, C.c_size_t]),
+    ("isp_scene_lens_shading", C.c_int, [P(SceneConfig), C.c_int32, C.c_int32, P(C.c_float)]),
+]
+
+
+def bind(lib, prototypes=PROTOTYPES):
+    """Sets restype/argtypes of every prototype on `lib`; raises if one is missing."""
+    for name, restype, argtypes in prototypes:
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    return lib

@@ -1,0 +1,237 @@
+/*
+ * isp_c.h - the C ABI of the MI355X image signal processor.
+ *
+ * Raw Bayer mosaic in, display-ready RGB out, with optional histograms, a
+ * statistics vector and an auto-exposure suggestion, computed by hand-written
+ * gfx950 HIP kernels on caller-owned device buffers.
+ *
+ * Every entry point below replaces one entry of the reference's C++ interface
+ * (holoscan::isp, modules/isp/include/holoscan/isp/ headers); the comment on each
+ * names the reference declaration (file:line) it stands in for. The types keep
+ * the reference's field names, meanings and defaults: enums are uint8_t,
+ * std::optional<T> becomes a presence flag beside a T, std::vector<float>
+ * becomes a pointer plus a count that the library copies at create time.
+ *
+ * Errors: no exceptions cross this ABI. Every function that can fail returns
+ * an isp_status; isp_last_error() then names the first unusable field with the
+ * reference's own wording (std::invalid_argument -> ISP_INVALID_ARGUMENT,
+ * std::runtime_error -> ISP_FAILURE).
+ *
+ * Threading: one host thread per pipeline, as in the reference.
+ */
+#ifndef ISP_C_H_
+#define ISP_C_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+
+typedef enum isp_status {
+  ISP_OK = 0,
+  ISP_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+  ISP_FAILURE = 2,          /* std::runtime_error ("IspPipeline (cuda): ...") */
+  ISP_OUT_OF_MEMORY = 3,    /* admission refused: the HBM budget is spent (serving) */
+  ISP_NOT_READY = 4         /* serving: no finished frame to collect yet */
+} isp_status;
+
+/* ---- enums (isp_params.hpp:21-61) --------------------------------------- */
+
+enum { ISP_RGGB = 0, ISP_BGGR = 1, ISP_GRBG = 2, ISP_GBRG = 3 };            /* BayerPattern */
+enum { ISP_RAW_U8 = 0, ISP_RAW_U16 = 1, ISP_RAW_RAW10 = 2, ISP_RAW_RAW12 = 3 }; /* RawFormat */
+enum { ISP_DEMOSAIC_BILINEAR = 0, ISP_DEMOSAIC_MALVAR = 1 };                 /* DemosaicMethod */
+enum { ISP_WB_MANUAL = 0, ISP_WB_GRAY_WORLD = 1, ISP_WB_WHITE_PATCH = 2 };   /* WhiteBalanceMode */
+enum { ISP_TONE_SRGB = 0, ISP_TONE_GAMMA = 1, ISP_TONE_LINEAR = 2, ISP_TONE_LUT = 3 }; /* ToneCurve */
+enum { ISP_OUT_RGB8 = 0, ISP_OUT_RGBA8 = 1, ISP_OUT_RGB16 = 2, ISP_OUT_RGB32F = 3 };  /* OutputFormat */
+/* Backend (isp_params.hpp:57-61): ISP_BACKEND_GPU is the reference's kCuda. */
+enum { ISP_BACKEND_GPU = 0, ISP_BACKEND_CPU = 1, ISP_BACKEND_AUTO = 2 };
+/* MemoryLocation (isp_pipeline.hpp:32) */
+enum { ISP_LOCATION_HOST = 0, ISP_LOCATION_DEVICE = 1 };
+
+/* Channels of the mosaic (isp_params.hpp:65-68). */
+enum { ISP_CHANNEL_R = 0, ISP_CHANNEL_GR = 1, ISP_CHANNEL_GB = 2, ISP_CHANNEL_B = 3 };
+
+/* Tone table size of the built-in curves (isp_params.hpp:128). */
+#define ISP_TONE_LUT_SIZE 4097
+/* Histogram layout (isp_params.hpp:166-171): luma, red, green, blue rows. */
+#define ISP_HISTOGRAM_BINS 256
+#define ISP_HISTOGRAM_CHANNELS 4
+/* Extension: 1024-bin histograms of 10-bit output codes (RGB16 / RGB32F outputs only). */
+#define ISP_HISTOGRAM_BINS_WIDE 1024
+/* Statistics vector, layout version 1 (isp_params.hpp:178-197). */
+#define ISP_STATS_LAYOUT_VERSION 1
+#define ISP_STATS_SIZE 12
+enum {
+  ISP_STAT_MEAN_R = 0, ISP_STAT_MEAN_G = 1, ISP_STAT_MEAN_B = 2, ISP_STAT_MEAN_LUMA = 3,
+  ISP_STAT_AE_ERROR_EV = 4, ISP_STAT_EXPOSURE_SCALE = 5, ISP_STAT_SUGGESTED_EXPOSURE_NS = 6,
+  ISP_STAT_SUGGESTED_GAIN = 7, ISP_STAT_WB_GAIN_R = 8, ISP_STAT_WB_GAIN_G = 9,
+  ISP_STAT_WB_GAIN_B = 10, ISP_STAT_SATURATED_FRACTION = 11
+};
+
+/* ---- settings (isp_params.hpp:73-121) ----------------------------------- */
+
+/* LensShading (isp_params.hpp:73-79): gains [height][width][channels], each in (0, 16]. */
+typedef struct isp_lens_shading {
+  int32_t width;       /* grid points across, 2..256 */
+  int32_t height;      /* grid points down, 2..256 */
+  int32_t channels;    /* 1 or 4 (R, Gr, Gb, B) */
+  const float* gains;  /* width * height * channels values, copied at create */
+  int64_t gains_count; /* the vector's size, checked against width * height * channels */
+} isp_lens_shading;
+
+/* AutoExposureParams (isp_params.hpp:82-93). */
+typedef struct isp_auto_exposure {
+  float target;            /* 0.18 */
+  float damping;           /* 0.5 */
+  int64_t min_exposure_ns; /* 10'000 */
+  int64_t max_exposure_ns; /* 33'000'000 */
+  float min_gain;          /* 1 */
+  float max_gain;          /* 16 */
+} isp_auto_exposure;
+
+/* IspParams (isp_params.hpp:95-121). Fill with isp_params_init() first. */
+typedef struct isp_params {
+  float black_level;
+  uint8_t has_channel_black_levels; /* std::optional<std::array<float, 4>> */
+  float channel_black_levels[4];
+  float white_level; /* 0: 2^bit_depth - 1 */
+  uint8_t has_lens_shading; /* std::optional<LensShading> */
+  isp_lens_shading lens_shading;
+  uint8_t white_balance; /* ISP_WB_* */
+  float wb_gains[3];
+  float awb_smoothing;
+  float white_patch_percentile;
+  uint8_t demosaic; /* ISP_DEMOSAIC_* */
+  float ccm[9];
+  uint8_t tone_curve; /* ISP_TONE_* */
+  float gamma;
+  const float* lut; /* ISP_TONE_LUT: 2..65536 entries, copied at create */
+  int32_t lut_size;
+  float sharpen;
+  isp_auto_exposure auto_exposure;
+  /* Extension, no reference counterpart: histogram bins, 256 (the reference's
+   * fixed layout; 0 means 256) or ISP_HISTOGRAM_BINS_WIDE. */
+  int32_t histogram_bins;
+} isp_params;
+
+/* RawImageView (isp_params.hpp:132-140). `pitch` is the row stride in bytes. */
+typedef struct isp_raw_view {
+  const void* data;
+  int32_t width;  /* at least 2 */
+  int32_t height; /* at least 2 */
+  size_t pitch;
+  uint8_t format;    /* ISP_RAW_* */
+  int32_t bit_depth; /* 1..8 u8, 10 RAW10, 12 RAW12, else 1..16 */
+  uint8_t pattern;   /* ISP_RGGB.. */
+} isp_raw_view;
+
+/* RgbImageView (isp_params.hpp:143-147). */
+typedef struct isp_rgb_view {
+  void* data;
+  size_t pitch;
+  uint8_t format; /* ISP_OUT_* */
+} isp_rgb_view;
+
+/* StatsOutput (isp_params.hpp:151-154); either pointer may be null. */
+typedef struct isp_stats_out {
+  uint32_t* histogram; /* ISP_HISTOGRAM_CHANNELS x bins */
+  float* stats;        /* ISP_STATS_SIZE */
+} isp_stats_out;
+
+/* FrameExposure (isp_params.hpp:157-160); 0 is unknown. */
+typedef struct isp_frame_exposure {
+  int64_t exposure_ns;
+  float gain;
+} isp_frame_exposure;
+
+/* A hipStream_t, passed as a plain pointer; NULL is the null stream. */
+typedef void* isp_stream_t;
+
+typedef struct isp_pipeline isp_pipeline;
+
+/* ---- settings helpers --------------------------------------------------- */
+
+/* The default IspParams{} (isp_params.hpp:95-121). */
+void isp_params_init(isp_params* params);
+/* validate_params (isp_params.hpp:124, isp_params.cpp:85-164). */
+isp_status isp_validate_params(const isp_params* params);
+/* validate_raw (isp_params.hpp:236, isp_params.cpp:199-237). */
+isp_status isp_validate_raw(const isp_raw_view* raw, const isp_params* params);
+/* raw_row_bytes (isp_params.hpp:232, isp_params.cpp:184-197). */
+size_t isp_raw_row_bytes(uint8_t format, int32_t width);
+/* tone_lut (isp_params.hpp:129, isp_params.cpp:166-182): writes the table into
+ * `out` (capacity entries) and its size into *size; with out == NULL only the size. */
+isp_status isp_tone_lut(const isp_params* params, float* out, int32_t capacity, int32_t* size);
+/* output_channels / output_channel_bytes (isp_params.hpp:224-229). */
+int32_t isp_output_channels(uint8_t format);
+int32_t isp_output_channel_bytes(uint8_t format);
+/* backend_name .. parse_white_balance (isp_params.hpp:200-211): names are static
+ * strings; parse returns 1 and sets *value, or 0 for an unknown name. */
+const char* isp_backend_name(uint8_t backend);
+int isp_parse_backend(const char* name, uint8_t* value);
+const char* isp_demosaic_name(uint8_t method);
+int isp_parse_demosaic(const char* name, uint8_t* value);
+const char* isp_pattern_name(uint8_t pattern);
+int isp_parse_pattern(const char* name, uint8_t* value);
+const char* isp_white_balance_name(uint8_t mode);
+int isp_parse_white_balance(const char* name, uint8_t* value);
+/* probe_platform (isp_pipeline.hpp:35): 1 when a HIP device is usable. */
+int isp_probe_platform(void);
+/* resolve_auto / resolve_backend (isp_params.hpp:219-221, isp_pipeline.hpp:37). */
+uint8_t isp_resolve_auto(int gpu_device);
+uint8_t isp_resolve_backend(uint8_t backend);
+
+/* The message of the last failed call on this thread (thread-local), "" if none. */
+const char* isp_last_error(void);
+
+/* Verification (no reference counterpart): the host evaluation of the arithmetic
+ * the kernels' statistics epilogue runs (finish_stats, isp_math.hpp:387-425, with
+ * a double-double log2 / exp2), for comparison with the host libm's. */
+double isp_epilogue_log2(double x);
+double isp_epilogue_exp2(double y);
+void isp_epilogue_finish_stats(const uint64_t sums[4], int64_t pixels, const float gains[3],
+                               const isp_auto_exposure* ae, const isp_frame_exposure* exposure,
+                               float* out);
+
+/* ---- the pipeline (isp_pipeline.hpp:39-74) ------------------------------ */
+
+/* IspPipeline(Backend, IspParams) (isp_pipeline.hpp:44, isp_pipeline.cpp:37-46):
+ * validates, resolves ISP_BACKEND_AUTO, builds the tone table, uploads it and the
+ * lens shading grid, and allocates the white balance and statistics scratch on
+ * the current HIP device. This library holds only the GPU backend: ISP_BACKEND_CPU,
+ * or ISP_BACKEND_AUTO without a usable device, is ISP_INVALID_ARGUMENT. */
+isp_status isp_create(const isp_params* params, uint8_t backend, isp_pipeline** out);
+/* ~IspPipeline: waits for the frames already queued, then frees. */
+isp_status isp_destroy(isp_pipeline* pipeline);
+/* backend() / location() (isp_pipeline.hpp:50-52). */
+uint8_t isp_backend(const isp_pipeline* pipeline);
+uint8_t isp_location(const isp_pipeline* pipeline);
+/* reset() (isp_pipeline.hpp:58, isp_pipeline.cpp:58-63): waits for the frames
+ * queued, then forgets the white balance estimate. */
+isp_status isp_reset(isp_pipeline* pipeline);
+/* process(raw, out, stats, exposure, stream) (isp_pipeline.hpp:68-69,
+ * isp_pipeline.cpp:65-90): queues one frame on `stream`; all buffers in device
+ * memory; the frame's GPU work waits for the previous frame's on any stream.
+ * `stats` and `exposure` may be NULL ({} in the reference). */
+isp_status isp_process(isp_pipeline* pipeline, const isp_raw_view* raw, const isp_rgb_view* out,
+                       const isp_stats_out* stats, const isp_frame_exposure* exposure,
+                       isp_stream_t stream);
+
+/* Kernel timing (no reference counterpart; the reference times whole frames with
+ * CUDA events in its benchmark, tests/benchmark.cpp:137-142): between begin and
+ * end, every process() records hipEvents on its stream around the white balance
+ * statistics kernel and the main kernel; end waits for them and returns the
+ * summed kernel times in milliseconds and the number of frames. */
+isp_status isp_timing_begin(isp_pipeline* pipeline);
+isp_status isp_timing_end(isp_pipeline* pipeline, double* awb_ms, double* main_ms,
+                          int32_t* frames);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISP_C_H_ */

@@ -1,0 +1,54 @@
+/*
+ * isp_oracle.h - TEST INFRASTRUCTURE ONLY: the CPU oracle of the ISP.
+ *
+ * A plain-C restatement of the reference's CPU backend
+ * (modules/isp/src/cpu_pipeline.cpp, modules/isp/src/isp_math.hpp,
+ * modules/isp/src/isp_params.cpp:166-182), used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's CPU-baseline leg as the checker of
+ * the HIP path. It is never linked into, loaded by or called from the product
+ * library. Each function in isp_oracle.c cites the reference lines it follows.
+ *
+ * Parity pins: see the header of isp_oracle.c.
+ */
+#ifndef ISP_ORACLE_H_
+#define ISP_ORACLE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/isp_c.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_pipeline oracle_pipeline;
+
+/* IspPipeline(Backend::kCpu, params) without validation: copies the tables. */
+int oracle_create(const isp_params* params, oracle_pipeline** out);
+void oracle_destroy(oracle_pipeline* pipeline);
+/* Forgets the white balance estimate. */
+void oracle_reset(oracle_pipeline* pipeline);
+/* One frame in host memory (cpu_process); `threads` > 1 splits every pass
+ * into row bands (the sums are integers, so the result is the same). */
+int oracle_process(oracle_pipeline* pipeline, const isp_raw_view* raw, const isp_rgb_view* out,
+                   const isp_stats_out* stats, const isp_frame_exposure* exposure, int threads);
+/* The white balance state after the last frame: gains[3], valid. */
+void oracle_awb_state(const oracle_pipeline* pipeline, float gains[3], int* valid);
+
+/* Pieces, for unit tests against closed forms. */
+int oracle_tone_lut(const isp_params* params, float* out, int capacity);
+int oracle_mirror(int v, int n);
+int oracle_load_raw(const isp_raw_view* raw, int x, int y);
+/* finish_stats on given sums (isp_math.hpp:387-425). */
+void oracle_finish_stats(const uint64_t sums[4], int64_t pixels, const float gains[3],
+                         const isp_auto_exposure* ae, const isp_frame_exposure* exposure,
+                         float* out);
+/* histogram_percentile (isp_math.hpp:336-352) over a 1024-bin histogram. */
+float oracle_histogram_percentile(const uint32_t* hist, float percentile);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISP_ORACLE_H_ */

@@ -1,0 +1,130 @@
+"""Shared test cases: synthetic-camera frames and settings, as the reference's
+GPU suite builds them (modules/isp/tests/pipeline_test.cpp:45-88, :164-235),
+plus the seeded noisy scenes of the benchmark configurations."""
+import holoscan_isp as isp
+from holoscan_isp import _abi
+
+# SampleFormat table (pipeline_test.cpp:50-57): format, bits, black.
+FORMATS = [
+    (_abi.RAW_U8, 8, 16.0),
+    (_abi.RAW_U16, 10, 64.0),
+    (_abi.RAW_U16, 12, 256.0),
+    (_abi.RAW_U16, 16, 4096.0),
+    (_abi.RAW_RAW10, 10, 64.0),
+    (_abi.RAW_RAW12, 12, 256.0),
+]
+PATTERNS = [_abi.RGGB, _abi.BGGR, _abi.GRBG, _abi.GBRG]
+
+
+def camera(pattern, fmt, width=480, height=360, **kw):
+    """camera() (pipeline_test.cpp:75-88): per-channel black levels, WB gains, CCM, vignetting 0.4, frame 3."""
+    f, bits, black = fmt
+    step = black / 16.0
+    cfg = dict(width=width, height=height, pattern=pattern, format=f, bit_depth=bits,
+               black_levels=(black, black + step, black + 2 * step, black + 3 * step),
+               vignetting=0.4, frame=3)
+    cfg.update(kw)
+    return isp.synthetic_config(**cfg)
+
+
+def four_channel(params):
+    """The 4-channel lens shading variant of cuda_matches_cpu (pipeline_test.cpp:200-206)."""
+    s = params.lens_shading
+    four = []
+    for g in s.gains:
+        for c in range(4):
+            factor = _f32(1.0) + _f32(0.01) * _f32(c)  # float arithmetic, as 1.0F + 0.01F * c
+            four.append(float(_f32(g) * factor))
+    params.lens_shading = isp.LensShading(s.width, s.height, 4, four)
+    return params
+
+
+def _f32(v):
+    import numpy as np
+    return np.float32(v)
+
+
+def matches_cpu_cases():
+    """The cases of cuda_matches_cpu (pipeline_test.cpp:164-208): (name, config, out_format, tweak)."""
+    u12 = FORMATS[2]
+    cases = []
+    for method in (_abi.BILINEAR, _abi.MALVAR):
+        m = "bilinear" if method == _abi.BILINEAR else "malvar"
+
+        def plain(p, method=method):
+            p.demosaic = method
+
+        def sharpened(p, method=method):
+            p.demosaic = method
+            p.sharpen = 1.5
+
+        def odd(p, method=method):
+            p.demosaic = method
+            p.sharpen = 0.5
+
+        def gray(p, method=method):
+            p.demosaic = method
+            p.white_balance = _abi.WB_GRAY_WORLD
+            p.tone_curve = _abi.TONE_GAMMA
+
+        def patch(p, method=method):
+            p.demosaic = method
+            p.white_balance = _abi.WB_WHITE_PATCH
+            p.tone_curve = _abi.TONE_LUT
+            p.lut = [0.0, 0.6, 0.85, 1.0]
+            four_channel(p)
+
+        cases.append((m, camera(_abi.RGGB, u12, 200, 150), _abi.OUT_RGB8, plain))
+        cases.append((m + " sharpened", camera(_abi.GBRG, u12, 200, 150), _abi.OUT_RGBA8, sharpened))
+        cases.append((m + " odd size", camera(_abi.GRBG, FORMATS[4], 67, 35), _abi.OUT_RGB16, odd))
+        cases.append((m + " gray world", camera(_abi.BGGR, FORMATS[5], 160, 120), _abi.OUT_RGB32F, gray))
+        cases.append((m + " white patch, 4-channel shading, table", camera(_abi.RGGB, FORMATS[0], 160, 120),
+                      _abi.OUT_RGB8, patch))
+    return cases
+
+
+def baseline_scene(width, height, fmt, bits, pattern, black, seed, **kw):
+    return isp.scene_config(width=width, height=height, format=fmt, bit_depth=bits, pattern=pattern,
+                            black_level=black, seed=seed, **kw)
+
+
+BASELINE_CCM = (1.6, -0.4, -0.2, -0.3, 1.5, -0.2, -0.1, -0.5, 1.6)
+
+
+def config1(width=1920, height=1080, seed=1):
+    """1920x1080 RGGB u16 12-bit, BL 256, 17x13 LSC = 1/falloff, manual WB (2, 1, 1.6),
+    bilinear, no sharpen, CCM, gamma 2.2, RGB8 + histograms."""
+    sc = baseline_scene(width, height, _abi.RAW_U16, 12, _abi.RGGB, 256.0, seed)
+    p = isp.IspParams(black_level=256.0, lens_shading=isp.scene_lens_shading(sc, 17, 13),
+                      white_balance=_abi.WB_MANUAL, wb_gains=(2.0, 1.0, 1.6), demosaic=_abi.BILINEAR,
+                      ccm=BASELINE_CCM, tone_curve=_abi.TONE_GAMMA, gamma=2.2)
+    return sc, p, _abi.OUT_RGB8
+
+
+def config2(width=3840, height=2160, seed=2):
+    """3840x2160 GRBG RAW12, BL 240, 17x13 LSC, gray world, Malvar, sharpen 0.5, CCM, sRGB,
+    RGBA8, histograms + AE."""
+    sc = baseline_scene(width, height, _abi.RAW_RAW12, 12, _abi.GRBG, 240.0, seed)
+    p = isp.IspParams(black_level=240.0, lens_shading=isp.scene_lens_shading(sc, 17, 13),
+                      white_balance=_abi.WB_GRAY_WORLD, demosaic=_abi.MALVAR, sharpen=0.5,
+                      ccm=BASELINE_CCM, tone_curve=_abi.TONE_SRGB)
+    return sc, p, _abi.OUT_RGBA8
+
+
+def srgb16_lut():
+    """The 16-bit tone curve: a 65536-entry sRGB table (ToneCurve::kLut)."""
+    import numpy as np
+    x = np.arange(65536, dtype=np.float64) / 65535.0
+    y = np.where(x <= 0.0031308, 12.92 * x, 1.055 * np.power(x, 1.0 / 2.4) - 0.055)
+    return np.clip(y, 0.0, 1.0).astype(np.float32)
+
+
+def config5(width=8192, height=6144, seed=5):
+    """8192x6144 RGGB u16 16-bit, BL 1024, white patch, Malvar, sharpen, CCM, 16-bit tone
+    curve, RGB16 + 1024-bin histograms."""
+    sc = baseline_scene(width, height, _abi.RAW_U16, 16, _abi.RGGB, 1024.0, seed)
+    p = isp.IspParams(black_level=1024.0, lens_shading=isp.scene_lens_shading(sc, 17, 13),
+                      white_balance=_abi.WB_WHITE_PATCH, demosaic=_abi.MALVAR, sharpen=0.5,
+                      ccm=BASELINE_CCM, tone_curve=_abi.TONE_LUT, lut=srgb16_lut(),
+                      histogram_bins=1024)
+    return sc, p, _abi.OUT_RGB16

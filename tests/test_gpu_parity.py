@@ -1,0 +1,155 @@
+"""The HIP pipeline against the CPU oracle, bit for bit: images, histograms and
+the whole statistics vector (the auto-exposure fields included), on the
+reference GPU suite's cases (modules/isp/tests/pipeline_test.cpp:164-235), a
+matrix of every pattern, sample format, demosaic, sharpening, white balance
+mode, tone curve and output format, odd and tiny sizes, padded rows, and the
+benchmark configurations' seeded noisy scenes."""
+import itertools
+
+import numpy as np
+import pytest
+
+import holoscan_isp as isp
+from holoscan_isp import _abi
+
+import cases
+import gpu_support as gs
+from oracle_lib import OraclePipeline
+
+pytestmark = pytest.mark.gpu
+
+EXPOSURE = (10_000_000, 2.0)
+
+
+def compare(name, g, h):
+    assert gs.mismatches(g, h) == 0, f"{name}: {gs.mismatches(g, h)} bytes differ"
+    if h.histogram is not None:
+        assert np.array_equal(g.histogram, h.histogram), f"{name}: histograms differ"
+        assert gs.same_stats(g, h), f"{name}: stats {g.stats} vs {h.stats}"
+
+
+@pytest.mark.parametrize("case", cases.matches_cpu_cases(), ids=lambda c: c[0])
+def test_matches_oracle_reference_cases(torch_gpu, case):
+    """cuda_matches_cpu (pipeline_test.cpp:164-235), AE fields bit-exact too."""
+    name, config, out_format, tweak = case
+    frame = isp.render_synthetic(config)
+    params = isp.synthetic_params(config, 9)
+    tweak(params)
+    gpu = isp.IspPipeline(params)
+    cpu = OraclePipeline(params)
+    for _ in range(2):  # two frames, so the white balance smoothing runs too
+        g = gs.run(gpu, frame, out_format, True, EXPOSURE)
+        h = cpu.process(frame, out_format, True, EXPOSURE)
+        compare(name, g, h)
+
+
+MATRIX = list(itertools.product(range(len(cases.FORMATS)), cases.PATTERNS, (_abi.BILINEAR, _abi.MALVAR),
+                                 (0.0, 0.7), (_abi.WB_MANUAL, _abi.WB_GRAY_WORLD, _abi.WB_WHITE_PATCH)))
+
+
+@pytest.mark.parametrize("fi,pattern,method,sharpen,wb", MATRIX)
+def test_matrix(torch_gpu, fi, pattern, method, sharpen, wb):
+    config = cases.camera(pattern, cases.FORMATS[fi], 70, 38)
+    frame = isp.render_synthetic(config)
+    params = isp.synthetic_params(config, 5)
+    params.demosaic, params.sharpen, params.white_balance = method, sharpen, wb
+    params.tone_curve = [_abi.TONE_SRGB, _abi.TONE_GAMMA, _abi.TONE_LINEAR][(fi + pattern) % 3]
+    out_format = (fi + pattern + method) % 4
+    gpu = isp.IspPipeline(params)
+    cpu = OraclePipeline(params)
+    for _ in range(2):
+        compare("matrix", gs.run(gpu, frame, out_format, True, EXPOSURE),
+                cpu.process(frame, out_format, True, EXPOSURE))
+
+
+def flat(width, height, r, g, b, pattern=_abi.RGGB):
+    """A flat 16-bit frame (pipeline_test.cpp:459-482)."""
+    cfa = [[0, 1, 2, 3], [3, 2, 1, 0], [1, 0, 3, 2], [2, 3, 0, 1]][pattern]
+    img = np.zeros((height, width), dtype=np.uint16)
+    for y in range(height):
+        for x in range(width):
+            ch = cfa[((y & 1) << 1) | (x & 1)]
+            img[y, x] = r if ch == 0 else b if ch == 3 else g
+    return isp.RawFrame(img.view(np.uint8).ravel(), width, height, 2 * width, _abi.RAW_U16, 16, pattern)
+
+
+@pytest.mark.parametrize("size", [(2, 2), (3, 5), (33, 17), (5, 2), (2, 9), (131, 3)])
+@pytest.mark.parametrize("pattern", cases.PATTERNS)
+@pytest.mark.parametrize("method", [_abi.BILINEAR, _abi.MALVAR])
+def test_tiny_sizes(torch_gpu, size, pattern, method):
+    """edge_cases (pipeline_test.cpp:637-661): down to 2x2, every pattern, sharpened."""
+    w, h = size
+    frame = flat(w, h, 9000, 21000, 15000, pattern)
+    params = isp.IspParams(tone_curve=_abi.TONE_LINEAR, white_level=65535.0, sharpen=1.0, demosaic=method)
+    g = gs.run(isp.IspPipeline(params), frame, _abi.OUT_RGB8, True)
+    c = OraclePipeline(params).process(frame, _abi.OUT_RGB8, True)
+    compare(f"{w}x{h}", g, c)
+
+
+def test_random_mosaics(torch_gpu):
+    """LCG mosaics (cpu_reference_test.cpp:57-66) at odd sizes, every option on."""
+    rng = np.random.default_rng(12345)
+    for w, h in [(13, 9), (6, 5), (97, 41), (257, 66), (64, 32), (65, 33)]:
+        img = rng.integers(0, 4096, size=(h, w), dtype=np.uint16)
+        frame = isp.RawFrame(img.view(np.uint8).ravel(), w, h, 2 * w, _abi.RAW_U16, 12, _abi.GBRG)
+        for method, sharpen, wb, out in [(0, 0.0, 0, 0), (1, 2.0, 1, 1), (1, 0.3, 2, 2), (0, 4.0, 1, 3)]:
+            params = isp.IspParams(channel_black_levels=(64.0, 60.0, 70.0, 50.0), wb_gains=(1.5, 1.0, 1.25),
+                                   demosaic=method, sharpen=sharpen, white_balance=wb,
+                                   ccm=(0.9, 0.2, -0.1, 0.05, 0.8, 0.15, -0.2, 0.1, 1.1))
+            compare(f"{w}x{h}", gs.run(isp.IspPipeline(params), frame, out, True, EXPOSURE),
+                    OraclePipeline(params).process(frame, out, True, EXPOSURE))
+
+
+def test_padded_rows(torch_gpu):
+    """padded_rows (pipeline_test.cpp:384-401): padding beyond the samples changes nothing."""
+    for fi in (0, 2, 5):
+        config = cases.camera(_abi.GBRG, cases.FORMATS[fi], 75, 41)
+        packed = isp.render_synthetic(config)
+        pitch = packed.pitch + 38
+        data = np.full(pitch * config.height, 0xAB, dtype=np.uint8)
+        for y in range(config.height):
+            data[y * pitch:y * pitch + packed.pitch] = packed.data[y * packed.pitch:(y + 1) * packed.pitch]
+        padded = isp.RawFrame(data, packed.width, packed.height, pitch, packed.format, packed.bit_depth,
+                              packed.pattern)
+        pipeline = isp.IspPipeline(isp.synthetic_params(config, 5))
+        a = gs.run(pipeline, packed, _abi.OUT_RGB8)
+        b = gs.run(pipeline, padded, _abi.OUT_RGB8)
+        compare("padded", a, b)
+        compare("padded vs oracle", b, OraclePipeline(isp.synthetic_params(config, 5)).process(padded))
+
+
+@pytest.mark.parametrize("which", ["config1", "config2", "config5"])
+def test_baseline_configs_reduced(torch_gpu, which):
+    """The benchmark configurations' pipelines on their seeded noisy scenes at reduced sizes."""
+    make = getattr(cases, which)
+    for w, h in [(320, 180), (258, 130)]:
+        sc, params, out_format = make(w, h)
+        frame = isp.render_scene(sc)
+        gpu = isp.IspPipeline(params)
+        cpu = OraclePipeline(params)
+        for _ in range(2):
+            compare(which, gs.run(gpu, frame, out_format, True, EXPOSURE),
+                    cpu.process(frame, out_format, True, EXPOSURE, threads=8))
+
+
+@pytest.mark.parametrize("which", ["config1", "config2"])
+def test_baseline_configs_full_size(torch_gpu, which):
+    """Configs 1 and 2 at full size (1920x1080 u16, 3840x2160 RAW12), two frames."""
+    sc, params, out_format = getattr(cases, which)()
+    frame = isp.render_scene(sc)
+    gpu = isp.IspPipeline(params)
+    cpu = OraclePipeline(params)
+    for _ in range(2):
+        compare(which, gs.run(gpu, frame, out_format, True, EXPOSURE),
+                cpu.process(frame, out_format, True, EXPOSURE, threads=16))
+
+
+def test_determinism(torch_gpu):
+    """determinism (pipeline_test.cpp:621-634): same bytes on every run and instance."""
+    config = cases.camera(_abi.RGGB, cases.FORMATS[2], 640, 480)
+    frame = isp.render_synthetic(config)
+    params = isp.synthetic_params(config)
+    params.white_balance, params.sharpen = _abi.WB_GRAY_WORLD, 1.0
+    a = gs.run(isp.IspPipeline(params), frame)
+    for _ in range(3):
+        compare("again", gs.run(isp.IspPipeline(params), frame), a)
