@@ -369,6 +369,7 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
   const bool want_hist = f.histogram != nullptr;
   const bool want_stats = f.stats != nullptr;
   const int bins = f.bins;
+  const bool aligned4 = ((reinterpret_cast<unsigned long long>(f.out) | static_cast<unsigned long long>(f.out_pitch)) & 3ULL) == 0;
   float gains[3];
   if (f.wb_mode != ISP_WB_MANUAL) {
     for (int k = 0; k < 3; ++k) gains[k] = s->wb.gains[k];
@@ -500,7 +501,12 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
                 packed |= v << (8 * k);
                 q[k] = static_cast<int>(v);
               }
-              reinterpret_cast<unsigned int*>(row)[x] = packed;
+              if (aligned4) {
+                reinterpret_cast<unsigned int*>(row)[x] = packed;
+              } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) row[4 * x + k] = static_cast<std::uint8_t>(packed >> (8 * k));
+              }
               break;
             }
             case ISP_OUT_RGB16:
