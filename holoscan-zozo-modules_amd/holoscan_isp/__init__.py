@@ -32,6 +32,12 @@ def lib():
         if not os.path.exists(LIBRARY):
             raise RuntimeError(f"{LIBRARY} is missing: build it with `make -C {PACKAGE_DIR}` "
                                "(or __graft_entry__.build())")
+        # torch (device memory and streams) must load its HIP runtime first: the
+        # library then binds to the same libamdhip64.so.7, so one runtime serves both.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = _abi.bind(C.CDLL(LIBRARY))
     return _lib
 
