@@ -1,0 +1,61 @@
+"""The C ABI without a GPU: the library loads, exports every function
+include/*.h declares, and the ctypes mirror's struct layouts equal the C
+compiler's."""
+import os
+import re
+import subprocess
+
+import ctypes as C
+
+import holoscan_isp as isp
+from holoscan_isp import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
+
+
+def declared_functions():
+    names = set()
+    for path in HEADERS:
+        text = open(path).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(isp_\w+)\s*\(", text, flags=re.M):
+            if not text[m.start():m.end()].lstrip().startswith(("typedef", "#")):
+                names.add(m.group(1))
+    return names
+
+
+def test_library_loads_and_exports_every_declared_function():
+    lib = isp.lib()
+    names = declared_functions()
+    assert len(names) > 30
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {p[0] for p in _abi.PROTOTYPES}
+    assert names <= bound, sorted(names - bound)
+
+
+STRUCTS = {"isp_params": _abi.Params, "isp_lens_shading": _abi.LensShading,
+           "isp_auto_exposure": _abi.AutoExposure, "isp_raw_view": _abi.RawView, "isp_rgb_view": _abi.RgbView,
+           "isp_stats_out": _abi.StatsOut, "isp_frame_exposure": _abi.FrameExposure,
+           "isp_synthetic_config": _abi.SyntheticConfig, "isp_scene_config": _abi.SceneConfig}
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "isp_synthetic.h"', 'int main(void) {']
+    for cname, py in STRUCTS.items():
+        src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            src.append(f'printf("{cname}.{field} %zu\\n", offsetof({cname}, {field}));')
+    src.append("return 0; }")
+    (tmp_path / "layout.c").write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(tmp_path / "layout.c"), "-o",
+                    str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.splitlines())
+    for cname, py in STRUCTS.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert int(got[f"{cname}.{field}"]) == getattr(py, field).offset, (cname, field)
