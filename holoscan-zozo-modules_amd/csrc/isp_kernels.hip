@@ -2,23 +2,40 @@
 //
 // Two launches per frame, one when the white balance is manual:
 //
-//  awb_kernel   white balance statistics over complete unsaturated 2x2 quads
-//               (reference: white_balance_quad isp_math.hpp:311-332, awb_kernel
-//               cuda_pipeline.cu:119-158) with an LDS-privatised 3 x 1024
-//               white-patch histogram; the last block to finish turns the sums
-//               into gains (finish_white_balance, isp_math.hpp:357-384) in the
-//               pipeline's device state and clears the sums for the next frame.
+//  awb_kernel   white balance statistics (reference: white_balance_quad,
+//               isp_math.hpp:311-332; awb_kernel, cuda_pipeline.cu:119-158).
+//               Persistent blocks walk 64x64 tiles; each thread takes 4
+//               complete 2x2 quads from two coalesced row-chunk loads, skips
+//               quads with a saturated sample, and sums the linearised values
+//               (2^-20 units) per colour in registers; white patch adds an
+//               LDS-privatised 3 x 1024 histogram. The last block to finish
+//               turns the sums into gains (finish_white_balance,
+//               isp_math.hpp:357-384) in the pipeline's device state and clears
+//               the sums for the next frame.
 //
-//  main_kernel  persistent blocks walk the frame's tiles: unpack + black level +
-//               lens shading + white balance into an LDS tile with a mirrored
-//               halo, demosaic (bilinear or Malvar-He-Cutler, only the two
-//               formulas each CFA site needs), unsharp mask over an LDS ring of
-//               demosaiced RGB, colour matrix, tone table, store, and the
-//               statistics (2^-20 fixed sums, saturated count) and 4 x bins
-//               histograms of the written codes in registers and LDS. The last
-//               block writes the histograms and the statistics vector
+//  main_kernel  persistent blocks walk 64x64 output tiles:
+//               1. coalesced chunk loads of the raw rows (RAW12: 8 samples from
+//                  3 dwords, RAW10: 16 from 5, u16: 8 from 4, u8: 8 from 2),
+//                  unpack, black level, lens shading, white balance and clip
+//                  into an LDS tile with a mirrored halo;
+//               2. with sharpening, Malvar / bilinear demosaic of the tile and
+//                  a one-pixel ring into LDS (4x2 pixels per thread, windows
+//                  read as 128-bit LDS loads, only the two formulas each CFA
+//                  site needs);
+//               3. per thread a 4x4 block of output pixels: unsharp mask (or
+//                  the demosaic itself), colour matrix, tone table, one vector
+//                  store per row, the statistics sums in registers and the
+//                  4 x bins histograms of the written codes in LDS.
+//               The last block writes the histograms and the statistics vector
 //               (finish_stats, isp_math.hpp:387-425, with the exact log2/exp2
 //               of isp_ae.hpp) and clears the scratch.
+//
+// Lens shading: the reference's per-sample bilinear interpolation
+// (lens_shading_gain, isp_math.hpp:121-140) is top + (bottom - top) * ty with
+// top and bottom depending only on the column and the grid row. Each tile
+// tabulates top and bottom - top per column for the (at most two) grid rows its
+// sample rows fall in, so a sample costs one multiply-add pair and gets the
+// same float result.
 //
 // Every float expression keeps the reference's operation order
 // (isp_math.hpp:121-305) and the library is built with -ffp-contract=off, so
@@ -37,6 +54,26 @@ namespace isp_amd {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kTile = 64;  // output tile edge (main and white balance passes)
+
+struct alignas(4) W3 {
+  unsigned int v[3];
+};
+struct alignas(4) W5 {
+  unsigned int v[5];
+};
+struct alignas(8) W2 {
+  unsigned int v[2];
+};
+struct alignas(16) W4 {
+  unsigned int v[4];
+};
+struct alignas(16) F4 {
+  float v[4];
+};
+struct alignas(8) F2 {
+  float v[2];
+};
 
 __device__ __forceinline__ int mirror_any(int v, int n) {
   const int period = 2 * (n - 1);
@@ -44,7 +81,7 @@ __device__ __forceinline__ int mirror_any(int v, int n) {
   if (v < 0) v += period;
   return v < n ? v : period - v;
 }
-// mirror (isp_math.hpp:82-91), with the in-range case first.
+// mirror (isp_math.hpp:82-91), with the in-range and one-reflection cases first.
 __device__ __forceinline__ int mirror_near(int v, int n) {
   if (static_cast<unsigned>(v) < static_cast<unsigned>(n)) return v;
   if (v < 0 && -v < n) return -v;
@@ -52,22 +89,19 @@ __device__ __forceinline__ int mirror_near(int v, int n) {
   return mirror_any(v, n);
 }
 
-__device__ __forceinline__ int channel_at(int cfa, int x, int y) {
-  return (cfa >> (2 * (((y & 1) << 1) | (x & 1)))) & 3;
-}
+__device__ __forceinline__ int cfa_channel(int cfa, int p) { return (cfa >> (2 * p)) & 3; }
 __device__ __forceinline__ int color_of(int channel) {
   return channel == ISP_CHANNEL_R ? 0 : channel == ISP_CHANNEL_B ? 2 : 1;
 }
 __device__ __forceinline__ float clamp01(float v) { return v < 0.0F ? 0.0F : v > 1.0F ? 1.0F : v; }
 
-// load_raw (isp_math.hpp:101-119).
-__device__ __forceinline__ int load_raw(const FrameDesc& f, int x, int y) {
-  const std::uint8_t* row = f.raw + static_cast<long long>(y) * f.raw_pitch;
-  switch (f.raw_format) {
+// load_raw (isp_math.hpp:101-119), byte-wise: the edges and unaligned frames.
+__device__ __forceinline__ int load_raw(const std::uint8_t* row, int format, int x) {
+  switch (format) {
     case ISP_RAW_U8:
       return row[x];
     case ISP_RAW_U16:
-      return reinterpret_cast<const std::uint16_t*>(row)[x];
+      return row[2 * x] | (row[2 * x + 1] << 8);
     case ISP_RAW_RAW10: {
       const std::uint8_t* g = row + 5 * (x >> 2);
       const int k = x & 3;
@@ -81,7 +115,83 @@ __device__ __forceinline__ int load_raw(const FrameDesc& f, int x, int y) {
   }
 }
 
-// lens_shading_gain (isp_math.hpp:121-140); f.lsc is not null.
+template <int kFmt>
+struct Chunk {
+  // Samples per aligned chunk load.
+  static constexpr int kSamples = kFmt == ISP_RAW_RAW10 ? 16 : 8;
+};
+
+__device__ __forceinline__ unsigned int byte_of(const unsigned int* w, int i) {
+  return (w[i >> 2] >> (8 * (i & 3))) & 0xFFU;
+}
+
+// kSamples samples at x (a multiple of kSamples) of an aligned row.
+template <int kFmt>
+__device__ __forceinline__ void load_chunk(const std::uint8_t* row, int x, int* s) {
+  if constexpr (kFmt == ISP_RAW_U8) {
+    const W2 w = *reinterpret_cast<const W2*>(row + x);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = static_cast<int>(byte_of(w.v, k));
+  } else if constexpr (kFmt == ISP_RAW_U16) {
+    const W4 w = *reinterpret_cast<const W4*>(row + 2 * x);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      s[2 * k] = static_cast<int>(w.v[k] & 0xFFFFU);
+      s[2 * k + 1] = static_cast<int>(w.v[k] >> 16);
+    }
+  } else if constexpr (kFmt == ISP_RAW_RAW12) {
+    const W3 w = *reinterpret_cast<const W3*>(row + 3 * (x >> 1));
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const unsigned int lo = byte_of(w.v, 3 * g + 2);
+      s[2 * g] = static_cast<int>((byte_of(w.v, 3 * g) << 4) | (lo & 15U));
+      s[2 * g + 1] = static_cast<int>((byte_of(w.v, 3 * g + 1) << 4) | (lo >> 4));
+    }
+  } else {
+    const W5 w = *reinterpret_cast<const W5*>(row + 5 * (x >> 2));
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const unsigned int lo = byte_of(w.v, 5 * g + 4);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[4 * g + k] = static_cast<int>((byte_of(w.v, 5 * g + k) << 2) | ((lo >> (2 * k)) & 3U));
+      }
+    }
+  }
+}
+
+// Wave64 sum of a 64-bit value.
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+  for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset, 64);
+  return v;
+}
+
+__device__ __forceinline__ unsigned int atomic_load_u32(unsigned int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long atomic_load_u64(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Whether this block is the last of the grid to get here; publishes the
+// block's earlier writes and atomics first.
+__device__ __forceinline__ bool last_block(unsigned int* done) {
+  __shared__ int is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int ticket = atomicAdd(done, 1U);
+    is_last = ticket == gridDim.x - 1 ? 1 : 0;
+  }
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last != 0;
+}
+
+// ---- lens shading ---------------------------------------------------------
+
+// lens_shading_gain (isp_math.hpp:121-140), per sample: the slow path for
+// tiles whose rows span more than two grid rows.
 __device__ __forceinline__ float lsc_gain(const FrameDesc& f, int x, int y, int channel) {
   const float fx = static_cast<float>(x) * f.lsc_sx;
   const float fy = static_cast<float>(y) * f.lsc_sy;
@@ -99,51 +209,92 @@ __device__ __forceinline__ float lsc_gain(const FrameDesc& f, int x, int y, int 
   return top + (bottom - top) * ty;
 }
 
-// linearize (isp_math.hpp:144-148).
-__device__ __forceinline__ float linearize(const FrameDesc& f, int raw, int x, int y, int channel) {
-  float v = (static_cast<float>(raw) - f.black[channel]) * f.inv_range[channel];
-  v = v > 0.0F ? v : 0.0F;
-  if (f.lsc != nullptr) v = v * lsc_gain(f, x, y, channel);
-  return v;
+__device__ __forceinline__ int lsc_row(const FrameDesc& f, int y, float* ty) {
+  const float fy = static_cast<float>(y) * f.lsc_sy;
+  int iy = static_cast<int>(fy);
+  iy = iy > f.lsc_height - 2 ? f.lsc_height - 2 : iy;
+  *ty = fy - static_cast<float>(iy);
+  return iy;
 }
 
-// Wave64 sum of a 64-bit value.
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-  for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset, 64);
-  return v;
+// The per-tile lens shading table: for sample columns cx0 .. cx0 + kCols - 1,
+// both row parities and grid rows iy0, iy0 + 1: (top, bottom - top).
+template <int kCols>
+struct LscTable {
+  F2 e[2][2][kCols];  // [grid row - iy0][row parity][column]
+};
+
+template <int kCols>
+__device__ void build_lsc_table(const FrameDesc& f, int cx0, int iy0, LscTable<kCols>& t) {
+  const int step = f.lsc_channels;
+  const int row = f.lsc_width * step;
+  for (int i = threadIdx.x; i < 4 * kCols; i += blockDim.x) {
+    const int c = i % kCols;
+    const int par = (i / kCols) & 1;
+    const int seg = i / (2 * kCols);
+    const int x = cx0 + c;
+    if (x < 0 || x >= f.width) continue;
+    const int iy = iy0 + seg > f.lsc_height - 2 ? f.lsc_height - 2 : iy0 + seg;
+    const float fx = static_cast<float>(x) * f.lsc_sx;
+    int ix = static_cast<int>(fx);
+    ix = ix > f.lsc_width - 2 ? f.lsc_width - 2 : ix;
+    const float tx = fx - static_cast<float>(ix);
+    const int channel = cfa_channel(f.cfa, (par << 1) | (x & 1));
+    const float* g = f.lsc + (iy * f.lsc_width + ix) * step + (step == 4 ? channel : 0);
+    const float top = g[0] + (g[step] - g[0]) * tx;
+    const float bottom = g[row] + (g[row + step] - g[row]) * tx;
+    t.e[seg][par][c].v[0] = top;
+    t.e[seg][par][c].v[1] = bottom - top;
+  }
 }
 
-__device__ __forceinline__ unsigned int atomic_load_u32(unsigned int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long atomic_load_u64(unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Whether this block is the last of the grid to reach this point; publishes
-// the block's earlier atomics first.
-__device__ __forceinline__ bool last_block(unsigned int* done) {
-  __shared__ int is_last;
-  __threadfence();
-  __syncthreads();
+// The range of grid rows the tile's (mirrored) sample rows [r0, r0 + n) fall in.
+__device__ void lsc_row_range(const FrameDesc& f, int r0, int n, int* lo, int* hi) {
+  __shared__ int s_lo;
+  __shared__ int s_hi;
   if (threadIdx.x == 0) {
-    const unsigned int ticket = atomicAdd(done, 1U);
-    is_last = ticket == gridDim.x - 1 ? 1 : 0;
+    s_lo = 1 << 30;
+    s_hi = -1;
   }
   __syncthreads();
-  if (is_last) __threadfence();
-  return is_last != 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    float ty;
+    const int iy = lsc_row(f, mirror_near(r0 + i, f.height), &ty);
+    atomicMin(&s_lo, iy);
+    atomicMax(&s_hi, iy);
+  }
+  __syncthreads();
+  *lo = s_lo;
+  *hi = s_hi;
+}
+
+// Per quad position p = (y & 1) * 2 + (x & 1): black, 1 / range, colour gain.
+struct QuadConst {
+  float black[4];
+  float inv[4];
+  float gain[4];
+};
+
+__device__ __forceinline__ QuadConst quad_consts(const FrameDesc& f, const float gains[3]) {
+  QuadConst q;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int ch = cfa_channel(f.cfa, p);
+    q.black[p] = f.black[ch];
+    q.inv[p] = f.inv_range[ch];
+    q.gain[p] = gains[color_of(ch)];
+  }
+  return q;
 }
 
 // ---- white balance statistics -------------------------------------------
 
 // histogram_percentile (isp_math.hpp:336-352) over hist[1024], by one wave:
-// each lane owns 16 consecutive bins; an exclusive prefix sum finds the bin.
+// each lane owns 16 consecutive bins; a prefix sum finds the bin.
 __device__ float percentile_wave(const unsigned int* hist, float percentile) {
   const int lane = threadIdx.x & 63;
   unsigned long long mine = 0;
   for (int b = 0; b < 16; ++b) mine += hist[lane * 16 + b];
-  // inclusive scan over lanes
   unsigned long long scan = mine;
   for (int offset = 1; offset < 64; offset <<= 1) {
     const unsigned long long up = __shfl_up(scan, offset, 64);
@@ -163,7 +314,6 @@ __device__ float percentile_wave(const unsigned int* hist, float percentile) {
     }
     below += h;
   }
-  // the lowest lane with a hit holds the first such bin
   const unsigned long long hits = __ballot(found != (1 << 30));
   if (hits == 0) return 1.0F;
   const int first = __ffsll(static_cast<long long>(hits)) - 1;
@@ -172,43 +322,114 @@ __device__ float percentile_wave(const unsigned int* hist, float percentile) {
   return static_cast<float>((static_cast<double>(bin) + w) / kAwbBins);
 }
 
-__global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratch* s) {
+template <int kFmt>
+__global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles,
+                                                       int fast_rows) {
+  constexpr int kC = Chunk<kFmt>::kSamples;
+  constexpr int kCols = kTile;
   __shared__ unsigned int hist[3 * kAwbBins];
   __shared__ unsigned long long part[kThreads / 64][4];
+  __shared__ LscTable<kCols> lsc_tab;
   const bool patch = f.wb_mode == ISP_WB_WHITE_PATCH;
   const int tid = threadIdx.x;
   if (patch) {
     for (int i = tid; i < 3 * kAwbBins; i += kThreads) hist[i] = 0U;
   }
-  __syncthreads();
+  QuadConst qc;
+  {
+    const float ones[3] = {1.0F, 1.0F, 1.0F};
+    qc = quad_consts(f, ones);
+  }
+  int colour[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) colour[p] = color_of(cfa_channel(f.cfa, p));
   const int qw = f.width / 2;
   const int qh = f.height / 2;
   unsigned long long local[4] = {0, 0, 0, 0};
-  for (int qy = blockIdx.x; qy < qh; qy += gridDim.x) {
-    for (int qx = tid; qx < qw; qx += kThreads) {
-      int raw[4];
-      bool ok = true;
+
+  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int tyi = tile / tiles_x;
+    const int x0 = (tile - tyi * tiles_x) * kTile;
+    const int y0 = tyi * kTile;
+    bool table = false;
+    int iy0 = 0;
+    __syncthreads();  // the previous tile's table readers are done
+    if (f.lsc != nullptr) {
+      int lo, hi;
+      lsc_row_range(f, y0, min(kTile, f.height - y0), &lo, &hi);
+      table = hi - lo <= 1;
+      iy0 = lo;
+      if (table) build_lsc_table<kCols>(f, x0, iy0, lsc_tab);
+      __syncthreads();
+    }
+    // Each task: kC columns x 2 rows = kC / 2 quads.
+    constexpr int kTasksX = kTile / kC;
+    constexpr int kTasks = kTasksX * (kTile / 2);
+    for (int task = tid; task < kTasks; task += kThreads) {
+      const int qr = task / kTasksX;
+      const int cx = x0 + (task - qr * kTasksX) * kC;
+      const int qy = (y0 >> 1) + qr;
+      if (qy >= qh || cx >= 2 * qw) continue;
+      int raw[2][kC];
+      const std::uint8_t* rows[2] = {f.raw + static_cast<long long>(2 * qy) * f.raw_pitch,
+                                     f.raw + static_cast<long long>(2 * qy + 1) * f.raw_pitch};
+      const bool whole = fast_rows != 0 && cx + kC <= f.width;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        raw[k] = load_raw(f, 2 * qx + (k & 1), 2 * qy + (k >> 1));
-        ok = ok && raw[k] < f.white;
-      }
-      if (!ok) continue;
+      for (int r = 0; r < 2; ++r) {
+        if (whole) {
+          load_chunk<kFmt>(rows[r], cx, raw[r]);
+        } else {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int x = 2 * qx + (k & 1);
-        const int y = 2 * qy + (k >> 1);
-        const int channel = channel_at(f.cfa, x, y);
-        float v = linearize(f, raw[k], x, y, channel);
-        v = v < 4.0F ? v : 4.0F;
-        const int color = color_of(channel);
-        local[color] += static_cast<unsigned int>(v * 1048576.0F + 0.5F);
-        if (patch) {
-          const int b = static_cast<int>(v * static_cast<float>(kAwbBins));
-          atomicAdd(&hist[color * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
+          for (int k = 0; k < kC; ++k) raw[r][k] = cx + k < 2 * qw ? load_raw(rows[r], kFmt, cx + k) : 0;
         }
       }
-      local[3] += 1;
+      float tyr[2] = {0.0F, 0.0F};
+      int seg[2] = {0, 0};
+      if (table) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) seg[r] = lsc_row(f, 2 * qy + r, &tyr[r]) - iy0;
+      }
+      unsigned int sum[3] = {0U, 0U, 0U};
+      unsigned int quads = 0U;
+#pragma unroll
+      for (int q = 0; q < kC / 2; ++q) {
+        const int qx = (cx >> 1) + q;
+        if (qx >= qw) continue;
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ok = ok && raw[k >> 1][2 * q + (k & 1)] < f.white;
+        if (!ok) continue;
+        quads += 1U;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int r = k >> 1;
+          const int c = 2 * q + (k & 1);
+          const int p = k;  // quad position: (y & 1) * 2 + (x & 1); quads start at even (x, y)
+          float v = (static_cast<float>(raw[r][c]) - qc.black[p]) * qc.inv[p];
+          v = v > 0.0F ? v : 0.0F;
+          if (f.lsc != nullptr) {
+            float gain;
+            if (table) {
+              const F2 e = lsc_tab.e[seg[r]][r][cx - x0 + c];
+              gain = e.v[0] + e.v[1] * tyr[r];
+            } else {
+              gain = lsc_gain(f, cx + c, 2 * qy + r, cfa_channel(f.cfa, p));
+            }
+            v = v * gain;
+          }
+          v = v < 4.0F ? v : 4.0F;
+          const int col = colour[p];
+          const unsigned int fixed = static_cast<unsigned int>(v * 1048576.0F + 0.5F);
+          sum[col] += fixed;
+          if (patch) {
+            const int b = static_cast<int>(v * static_cast<float>(kAwbBins));
+            atomicAdd(&hist[col * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) local[k] += sum[k];
+      local[3] += quads;
     }
   }
   const int wave = tid >> 6;
@@ -235,7 +456,9 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
   __shared__ unsigned long long quads_s;
   if (tid == 0) quads_s = atomic_load_u64(&s->awb.quads);
   if (patch) {
-    for (int i = tid; i < 3 * kAwbBins; i += kThreads) hist[i] = atomic_load_u32(&s->awb.hist[i / kAwbBins][i % kAwbBins]);
+    for (int i = tid; i < 3 * kAwbBins; i += kThreads) {
+      hist[i] = atomic_load_u32(&s->awb.hist[i / kAwbBins][i % kAwbBins]);
+    }
   }
   __syncthreads();
   if (quads_s != 0) {
@@ -269,7 +492,6 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
       st.valid = 1;
     }
   }
-  // Clear for the next frame.
   if (tid < 3) s->awb.sum[tid] = 0;
   if (tid == 3) s->awb.quads = 0;
   if (patch) {
@@ -280,46 +502,48 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
 
 // ---- the main pass ----------------------------------------------------------
 
-// Demosaic (isp_math.hpp:161-215) of the sample at p (an LDS pointer into a tile
-// of row stride `sw`) whose channel is `channel` (uniform across the wave).
-template <bool kMalvar>
-__device__ __forceinline__ void demosaic_at(const float* p, int sw, int channel, float rgb[3]) {
-  const float c = p[0];
-  const float n12 = p[-sw];
-  const float n32 = p[sw];
-  const float n21 = p[-1];
-  const float n23 = p[1];
-  const bool rb = channel == ISP_CHANNEL_R || channel == ISP_CHANNEL_B;
-  float first;   // at_green (R/B sites) or across (G sites)
-  float second;  // diagonal (R/B sites) or down (G sites)
+// The two demosaic formulas (isp_math.hpp:161-215) a CFA site needs, from a
+// window w[row][col] whose centre is w[cy][cx]: at an R or B site green and the
+// diagonal colour, at a G site the colour across and the colour down.
+template <bool kMalvar, int kRows, int kCols>
+__device__ __forceinline__ void demosaic_pair(const float (&w)[kRows][kCols], int cy, int cx, bool rb,
+                                              float* first, float* second) {
+  const float c = w[cy][cx];
+  const float n12 = w[cy - 1][cx];
+  const float n32 = w[cy + 1][cx];
+  const float n21 = w[cy][cx - 1];
+  const float n23 = w[cy][cx + 1];
   if constexpr (!kMalvar) {
     if (rb) {
-      first = (n12 + n32 + n21 + n23) * 0.25F;
-      second = (p[-sw - 1] + p[-sw + 1] + p[sw - 1] + p[sw + 1]) * 0.25F;
+      *first = (n12 + n32 + n21 + n23) * 0.25F;
+      *second = (w[cy - 1][cx - 1] + w[cy - 1][cx + 1] + w[cy + 1][cx - 1] + w[cy + 1][cx + 1]) * 0.25F;
     } else {
-      first = (n21 + n23) * 0.5F;
-      second = (n12 + n32) * 0.5F;
+      *first = (n21 + n23) * 0.5F;
+      *second = (n12 + n32) * 0.5F;
     }
   } else {
-    const float n02 = p[-2 * sw];
-    const float n42 = p[2 * sw];
-    const float n20 = p[-2];
-    const float n24 = p[2];
+    const float n02 = w[cy - 2][cx];
+    const float n42 = w[cy + 2][cx];
+    const float n20 = w[cy][cx - 2];
+    const float n24 = w[cy][cx + 2];
+    const float diagonal1 = w[cy - 1][cx - 1] + w[cy - 1][cx + 1] + w[cy + 1][cx - 1] + w[cy + 1][cx + 1];
     const float vertical2 = n02 + n42;
     if (rb) {
       const float axial1 = n12 + n32 + n21 + n23;
       const float axial2 = n02 + n42 + n20 + n24;
-      const float diagonal1 = p[-sw - 1] + p[-sw + 1] + p[sw - 1] + p[sw + 1];
-      first = clamp01((4.0F * c + 2.0F * axial1 - axial2) * 0.125F);
-      second = clamp01((6.0F * c + 2.0F * diagonal1 - 1.5F * axial2) * 0.125F);
+      *first = clamp01((4.0F * c + 2.0F * axial1 - axial2) * 0.125F);
+      *second = clamp01((6.0F * c + 2.0F * diagonal1 - 1.5F * axial2) * 0.125F);
     } else {
-      const float diagonal1 = p[-sw - 1] + p[-sw + 1] + p[sw - 1] + p[sw + 1];
       const float horizontal2 = n20 + n24;
-      first = clamp01((5.0F * c + 4.0F * (n21 + n23) - horizontal2 - diagonal1 + 0.5F * vertical2) * 0.125F);
-      second = clamp01((5.0F * c + 4.0F * (n12 + n32) - vertical2 - diagonal1 + 0.5F * horizontal2) * 0.125F);
+      *first = clamp01((5.0F * c + 4.0F * (n21 + n23) - horizontal2 - diagonal1 + 0.5F * vertical2) * 0.125F);
+      *second = clamp01((5.0F * c + 4.0F * (n12 + n32) - vertical2 - diagonal1 + 0.5F * horizontal2) * 0.125F);
     }
   }
-  switch (channel) {
+}
+
+// RGB of a site of channel `ch` (uniform) from its sample c and the pair above.
+__device__ __forceinline__ void assign_rgb(int ch, float c, float first, float second, float* rgb) {
+  switch (ch) {
     case ISP_CHANNEL_R:
       rgb[0] = c;
       rgb[1] = first;
@@ -343,194 +567,428 @@ __device__ __forceinline__ void demosaic_at(const float* p, int sw, int channel,
   }
 }
 
-// tone (isp_math.hpp:235-243).
-__device__ __forceinline__ float tone(const float* lut, int size, float x) {
+// tone (isp_math.hpp:235-243) on a table in global memory.
+__device__ __forceinline__ float tone(const float* __restrict__ lut, int size, float x) {
   const float f = x * static_cast<float>(size - 1);
   const int i = static_cast<int>(f);
   if (i >= size - 1) return lut[size - 1];
   const float t = f - static_cast<float>(i);
-  return lut[i] + t * (lut[i + 1] - lut[i]);
+  const float a = lut[i];
+  return a + t * (lut[i + 1] - a);
 }
 
-template <bool kMalvar, bool kSharpen, int kTileW, int kTileH>
-__global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles) {
-  constexpr int kRadius = kMalvar ? 2 : 1;
-  constexpr int kHalo = kRadius + (kSharpen ? 1 : 0);
-  constexpr int kSW = kTileW + 2 * kHalo;
-  constexpr int kSH = kTileH + 2 * kHalo;
-  constexpr int kRW = kSharpen ? kTileW + 2 : 1;
-  constexpr int kRH = kSharpen ? kTileH + 2 : 1;
-  __shared__ float samp[kSH * kSW];
-  __shared__ float ring[3][kRH * kRW];
-  __shared__ unsigned int hist[4 * kMaxHistBins];
+template <int kFmt, bool kMalvar, bool kSharpen>
+struct MainGeom {
+  static constexpr int kRadius = kMalvar ? 2 : 1;
+  static constexpr int kHalo = kRadius + (kSharpen ? 1 : 0);
+  static constexpr int kC = Chunk<kFmt>::kSamples;
+  // Sample tile: frame columns x0 - kHalo .. x0 + kTile + kHalo - 1 from LDS
+  // column 0; the stride leaves room for the 8-wide windows of the last blocks
+  // and keeps every window read 16-byte aligned.
+  static constexpr int kSW = (kTile + 2 * kHalo + 2 + 3) / 4 * 4;
+  static constexpr int kSH = kTile + 2 * kHalo;
+  // Ring of demosaiced RGB (tile plus one pixel), 4-wide blocks.
+  static constexpr int kRW = kSharpen ? (kTile + 2 + 3) / 4 * 4 : 4;
+  static constexpr int kRH = kSharpen ? kTile + 2 : 2;
+  // Lens shading table columns: the chunk-aligned span covering the sample columns.
+  static constexpr int kLscCols = (kTile + 2 * kHalo + 2 * kC - 2) / kC * kC;
+};
+
+template <int kFmt, bool kMalvar, bool kSharpen>
+__global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles,
+                                                        int fast_rows, int out_aligned) {
+  using G = MainGeom<kFmt, kMalvar, kSharpen>;
+  constexpr int kR = G::kRadius;
+  constexpr int kH = G::kHalo;
+  constexpr int kC = G::kC;
+  constexpr int kSW = G::kSW;
+  constexpr int kSH = G::kSH;
+  constexpr int kRW = G::kRW;
+  constexpr int kRH = G::kRH;
+  constexpr int kLscCols = G::kLscCols;
+  __shared__ __attribute__((aligned(16))) float samp[kSH * kSW];
+  __shared__ __attribute__((aligned(16))) float ring[3][kRH * kRW];
+  __shared__ unsigned int hist[4 * ISP_HISTOGRAM_BINS];
+  __shared__ LscTable<kLscCols> lsc_tab;
   __shared__ unsigned long long part[kThreads / 64][4];
 
   const int tid = threadIdx.x;
   const bool want_hist = f.histogram != nullptr;
   const bool want_stats = f.stats != nullptr;
   const int bins = f.bins;
-  const bool aligned4 = ((reinterpret_cast<unsigned long long>(f.out) | static_cast<unsigned long long>(f.out_pitch)) & 3ULL) == 0;
+  const bool wide = bins != ISP_HISTOGRAM_BINS;  // 1024-bin extension: global atomics
   float gains[3];
   if (f.wb_mode != ISP_WB_MANUAL) {
+#pragma unroll
     for (int k = 0; k < 3; ++k) gains[k] = s->wb.gains[k];
   } else {
+#pragma unroll
     for (int k = 0; k < 3; ++k) gains[k] = f.wb[k];
   }
-  if (want_hist) {
-    for (int i = tid; i < 4 * bins; i += kThreads) hist[i] = 0U;
+  const QuadConst qc = quad_consts(f, gains);
+  if (want_hist && !wide) {
+    for (int i = tid; i < 4 * ISP_HISTOGRAM_BINS; i += kThreads) hist[i] = 0U;
   }
   unsigned long long acc[4] = {0, 0, 0, 0};
   const int W = f.width;
   const int H = f.height;
+  const float* __restrict__ lut = f.lut;
+  const int lut_size = f.lut_size;
+  const float ccm[9] = {f.ccm[0], f.ccm[1], f.ccm[2], f.ccm[3], f.ccm[4], f.ccm[5], f.ccm[6], f.ccm[7], f.ccm[8]};
 
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const int ty = tile / tiles_x;
-    const int tx = tile - ty * tiles_x;
-    const int x0 = tx * kTileW;
-    const int y0 = ty * kTileH;
-    __syncthreads();
-    // 1. Samples of the tile and its halo, reflected at the frame's edges,
-    // preprocessed once (preprocess, isp_math.hpp:151-155).
+    const int tyi = tile / tiles_x;
+    const int x0 = (tile - tyi * tiles_x) * kTile;
+    const int y0 = tyi * kTile;
+    const int sx0 = x0 - kH;  // frame column of sample column 0
+    const int sy0 = y0 - kH;
+    const int cx0 = (sx0 >= 0 ? sx0 / kC : -((-sx0 + kC - 1) / kC)) * kC;  // chunk-aligned start
+    const bool edge_x = sx0 < 0 || x0 + kTile + kH > W;
+    __syncthreads();  // the previous tile's readers are done
+    bool table = false;
+    int iy0 = 0;
+    if (f.lsc != nullptr) {
+      int lo, hi;
+      lsc_row_range(f, sy0, kSH, &lo, &hi);
+      table = hi - lo <= 1;
+      iy0 = lo;
+      if (table) build_lsc_table<kLscCols>(f, cx0, iy0, lsc_tab);
+      __syncthreads();
+    }
+
+    // 1. Samples (preprocess, isp_math.hpp:151-155).
     unsigned int saturated = 0;
-    for (int i = tid; i < kSW * kSH; i += kThreads) {
-      const int sy = i / kSW;
-      const int sx = i - sy * kSW;
-      const int gx = x0 + sx - kHalo;
-      const int gy = y0 + sy - kHalo;
-      const int x = mirror_near(gx, W);
-      const int y = mirror_near(gy, H);
-      const int raw = load_raw(f, x, y);
-      const int channel = channel_at(f.cfa, x, y);
-      float v = linearize(f, raw, x, y, channel) * gains[color_of(channel)];
-      v = v < 1.0F ? v : 1.0F;
-      samp[i] = v;
-      if (sx >= kHalo && sx < kHalo + kTileW && sy >= kHalo && sy < kHalo + kTileH && gx < W && gy < H) {
-        saturated += raw >= f.white ? 1U : 0U;
+    const int n_chunks = (x0 + kTile + kH - cx0 + kC - 1) / kC;
+    for (int task = tid; task < kSH * n_chunks; task += kThreads) {
+      const int sr = task / n_chunks;
+      const int cx = cx0 + (task - sr * n_chunks) * kC;
+      const int yv = sy0 + sr;  // virtual row
+      const int y = mirror_near(yv, H);
+      const std::uint8_t* row = f.raw + static_cast<long long>(y) * f.raw_pitch;
+      int raw[kC];
+      if (fast_rows != 0 && cx >= 0 && cx + kC <= W) {
+        load_chunk<kFmt>(row, cx, raw);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kC; ++k) raw[k] = cx + k >= 0 && cx + k < W ? load_raw(row, kFmt, cx + k) : 0;
+      }
+      const int par = y & 1;
+      const float b0 = par ? qc.black[2] : qc.black[0];
+      const float b1 = par ? qc.black[3] : qc.black[1];
+      const float i0 = par ? qc.inv[2] : qc.inv[0];
+      const float i1 = par ? qc.inv[3] : qc.inv[1];
+      const float g0 = par ? qc.gain[2] : qc.gain[0];
+      const float g1 = par ? qc.gain[3] : qc.gain[1];
+      float tyr = 0.0F;
+      int seg = 0;
+      if (table) seg = lsc_row(f, y, &tyr) - iy0;
+      const bool count = want_stats && yv >= y0 && yv < y0 + kTile && yv < H;
+#pragma unroll
+      for (int k = 0; k < kC; ++k) {
+        const int x = cx + k;
+        const int col = x - sx0;
+        if (col < 0 || col >= kSW || x < 0 || x >= W) continue;
+        const bool odd = (k & 1) != 0;  // cx is even
+        float v = (static_cast<float>(raw[k]) - (odd ? b1 : b0)) * (odd ? i1 : i0);
+        v = v > 0.0F ? v : 0.0F;
+        if (f.lsc != nullptr) {
+          float gain;
+          if (table) {
+            const F2 e = lsc_tab.e[seg][par][x - cx0];
+            gain = e.v[0] + e.v[1] * tyr;
+          } else {
+            gain = lsc_gain(f, x, y, cfa_channel(f.cfa, (par << 1) | (x & 1)));
+          }
+          v = v * gain;
+        }
+        v = v * (odd ? g1 : g0);
+        v = v < 1.0F ? v : 1.0F;
+        samp[sr * kSW + col] = v;
+        if (count && x >= x0 && x < x0 + kTile) saturated += raw[k] >= f.white ? 1U : 0U;
       }
     }
     acc[3] += saturated;
+    if (edge_x) {
+      // Columns outside the frame take the sample of their reflection.
+      __syncthreads();
+      for (int i = tid; i < kSH * kSW; i += kThreads) {
+        const int sr = i / kSW;
+        const int col = i - sr * kSW;
+        const int x = sx0 + col;
+        if (x >= 0 && x < W) continue;
+        const int src = mirror_near(x, W) - sx0;
+        samp[i] = src >= 0 && src < kSW && src + sx0 < W ? samp[sr * kSW + src] : 0.0F;
+      }
+    }
     __syncthreads();
-    // 2. With sharpening, the RGB of the tile and a one-pixel ring, in 2x2
-    // quads starting at virtual pixel (x0 - 1, y0 - 1).
+
+    // 2. With sharpening: the demosaiced RGB of the tile and a one-pixel ring,
+    // ring pixel (rc, rr) = virtual pixel (x0 - 1 + rc, y0 - 1 + rr), in 4x2 blocks.
     if constexpr (kSharpen) {
-      constexpr int kQW = kRW / 2;
-      constexpr int kQuads = kQW * (kRH / 2);
-      for (int qi = tid; qi < kQuads; qi += kThreads) {
-        const int qy = qi / kQW;
-        const int qx = qi - qy * kQW;
-        const int rx = 2 * qx;
-        const int ry = 2 * qy;
+      constexpr int kBX = kRW / 4;
+      constexpr int kBlocks = kBX * (kRH / 2);
+      for (int b = tid; b < kBlocks; b += kThreads) {
+        const int by = b / kBX;
+        const int bx = b - by * kBX;
+        const int rc = 4 * bx;
+        const int rr = 2 * by;
+        // window: sample rows rr + kH - 1 - kR .. + 1 + kR, columns rc + kH - 1 - kR, 8 wide
+        constexpr int kWR = 2 + 2 * kR;
+        constexpr int kWC = 8;
+        float w[kWR][kWC];
+        const int base = (rr + kH - 1 - kR) * kSW + rc + kH - 1 - kR;
 #pragma unroll
-        for (int dy = 0; dy < 2; ++dy) {
+        for (int r = 0; r < kWR; ++r) {
+          const F4 a = *reinterpret_cast<const F4*>(&samp[base + r * kSW]);
+          const F4 c = *reinterpret_cast<const F4*>(&samp[base + r * kSW + 4]);
 #pragma unroll
-          for (int dx = 0; dx < 2; ++dx) {
-            // virtual pixel (x0 + rx + dx - 1, y0 + ry + dy - 1): parity (1 - dx, 1 - dy)
-            const int channel = (f.cfa >> (2 * (((1 - dy) << 1) | (1 - dx)))) & 3;
-            float rgb[3];
-            demosaic_at<kMalvar>(&samp[(ry + dy + kHalo - 1) * kSW + rx + dx + kHalo - 1], kSW, channel, rgb);
-            const int r = (ry + dy) * kRW + rx + dx;
-            ring[0][r] = rgb[0];
-            ring[1][r] = rgb[1];
-            ring[2][r] = rgb[2];
+          for (int k = 0; k < 4; ++k) {
+            w[r][k] = a.v[k];
+            w[r][4 + k] = c.v[k];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          float rgbs[4][3];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            // virtual pixel parity: ((1 + i) & 1, (1 + j) & 1)
+            const int ch = cfa_channel(f.cfa, (((1 + j) & 1) << 1) | ((1 + i) & 1));
+            const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
+            float first, second;
+            demosaic_pair<kMalvar>(w, kR + j, kR + i, rb, &first, &second);
+            assign_rgb(ch, w[kR + j][kR + i], first, second, rgbs[i]);
+          }
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            F4 o;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o.v[i] = rgbs[i][k];
+            *reinterpret_cast<F4*>(&ring[k][(rr + j) * kRW + rc]) = o;
           }
         }
       }
       __syncthreads();
     }
-    // 3. Each thread: a 4 x 2 block of output pixels.
-    constexpr int kBW = kTileW / 4;
-    constexpr int kBlocks = kBW * (kTileH / 2);
-    for (int b = tid; b < kBlocks; b += kThreads) {
-      const int by = b / kBW;
-      const int bx = b - by * kBW;
+
+    // 3. A 4x4 block of output pixels per thread.
+    {
+      const int bx = tid & 15;
+      const int by = tid >> 4;
+      const int px = 4 * bx;
+      const int py = 4 * by;
+      const int xo = x0 + px;
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const int py = 2 * by + r;
-        const int y = y0 + py;
+      for (int j = 0; j < 4; ++j) {
+        const int yo = y0 + py + j;
+        float rgb[4][3];
+        if constexpr (kSharpen) {
+          // ring rows py + j .. + 2, columns px .. px + 5
+          float m[3][3][6];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int px = 4 * bx + c;
-          const int x = x0 + px;
-          float rgb[3];
-          if constexpr (kSharpen) {
-            const int q = (py + 1) * kRW + px + 1;
+          for (int k = 0; k < 3; ++k) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              const float* src = &ring[k][(py + j + r) * kRW + px];
+              const F4 a = *reinterpret_cast<const F4*>(src);
+              const F2 b2 = *reinterpret_cast<const F2*>(src + 4);
+              m[k][r][0] = a.v[0];
+              m[k][r][1] = a.v[1];
+              m[k][r][2] = a.v[2];
+              m[k][r][3] = a.v[3];
+              m[k][r][4] = b2.v[0];
+              m[k][r][5] = b2.v[1];
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-              const float* m = &ring[k][q];
-              const float cc = m[0];
-              const float blur = (4.0F * cc + 2.0F * (m[-kRW] + m[kRW] + m[-1] + m[1]) +
-                                  (m[-kRW - 1] + m[-kRW + 1] + m[kRW - 1] + m[kRW + 1])) *
+              // sharpen (isp_math.hpp:219-227)
+              const float c = m[k][1][i + 1];
+              const float blur = (4.0F * c + 2.0F * (m[k][0][i + 1] + m[k][2][i + 1] + m[k][1][i] + m[k][1][i + 2]) +
+                                  (m[k][0][i] + m[k][0][i + 2] + m[k][2][i] + m[k][2][i + 2])) *
                                  0.0625F;
-              rgb[k] = clamp01(cc + f.sharpen * (cc - blur));
+              rgb[i][k] = clamp01(c + f.sharpen * (c - blur));
             }
-          } else {
-            const int channel = (f.cfa >> (2 * ((r << 1) | (c & 1)))) & 3;
-            demosaic_at<kMalvar>(&samp[(py + kHalo) * kSW + px + kHalo], kSW, channel, rgb);
           }
-          if (x >= W || y >= H) continue;
-          // correct_color (isp_math.hpp:229-233)
+        } else {
+          constexpr int kWR = 1 + 2 * kR;
+          constexpr int kWC = 8;
+          float w[kWR][kWC];
+          const int base = (py + j + kH - kR) * kSW + px + kH - kR;
+#pragma unroll
+          for (int r = 0; r < kWR; ++r) {
+            const F4 a = *reinterpret_cast<const F4*>(&samp[base + r * kSW]);
+            const F4 c = *reinterpret_cast<const F4*>(&samp[base + r * kSW + 4]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              w[r][k] = a.v[k];
+              w[r][4 + k] = c.v[k];
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int ch = cfa_channel(f.cfa, ((j & 1) << 1) | (i & 1));
+            const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
+            float first, second;
+            demosaic_pair<kMalvar>(w, kR, kR + i, rb, &first, &second);
+            assign_rgb(ch, w[kR][kR + i], first, second, rgb[i]);
+          }
+        }
+        if (yo >= H) continue;
+        // correct_color, to_fixed sums, tone, store, output codes (isp_math.hpp:229-305)
+        float t[4][3];
+        int q[4][3];
+        const bool valid[4] = {xo < W, xo + 1 < W, xo + 2 < W, xo + 3 < W};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
           float lin[3];
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            lin[k] = clamp01(f.ccm[3 * k] * rgb[0] + f.ccm[3 * k + 1] * rgb[1] + f.ccm[3 * k + 2] * rgb[2]);
+            lin[k] = clamp01(ccm[3 * k] * rgb[i][0] + ccm[3 * k + 1] * rgb[i][1] + ccm[3 * k + 2] * rgb[i][2]);
           }
-          if (want_stats) {
+          if (want_stats && valid[i]) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) acc[k] += static_cast<unsigned int>(lin[k] * 1048576.0F + 0.5F);
           }
-          float t[3];
 #pragma unroll
-          for (int k = 0; k < 3; ++k) t[k] = tone(f.lut, f.lut_size, lin[k]);
-          // store (isp_math.hpp:250-276) and output_codes (:279-301)
-          std::uint8_t* row = f.out + static_cast<long long>(y) * f.out_pitch;
-          int q[3];
-          switch (f.out_format) {
-            case ISP_OUT_RGB8:
+          for (int k = 0; k < 3; ++k) t[i][k] = tone(lut, lut_size, lin[k]);
+        }
+        std::uint8_t* orow = f.out + static_cast<long long>(yo) * f.out_pitch;
+        const bool whole = out_aligned != 0 && xo + 4 <= W;
+        switch (f.out_format) {
+          case ISP_OUT_RGB8: {
+            unsigned int byte[12];
 #pragma unroll
-              for (int k = 0; k < 3; ++k) {
-                const unsigned int v = static_cast<unsigned char>(t[k] * 255.0F + 0.5F);
-                row[3 * x + k] = static_cast<std::uint8_t>(v);
-                q[k] = static_cast<int>(v);
-              }
-              break;
-            case ISP_OUT_RGBA8: {
-              unsigned int packed = 0xFF000000U;
+            for (int i = 0; i < 4; ++i) {
 #pragma unroll
               for (int k = 0; k < 3; ++k) {
-                const unsigned int v = static_cast<unsigned char>(t[k] * 255.0F + 0.5F);
-                packed |= v << (8 * k);
-                q[k] = static_cast<int>(v);
+                const unsigned int v = static_cast<unsigned char>(t[i][k] * 255.0F + 0.5F);
+                q[i][k] = static_cast<int>(v);
+                byte[3 * i + k] = v;
               }
-              if (aligned4) {
-                reinterpret_cast<unsigned int*>(row)[x] = packed;
-              } else {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) row[4 * x + k] = static_cast<std::uint8_t>(packed >> (8 * k));
-              }
-              break;
             }
-            case ISP_OUT_RGB16:
+            if (whole) {
+              W3 o;
 #pragma unroll
-              for (int k = 0; k < 3; ++k) {
-                const unsigned int v = static_cast<std::uint16_t>(t[k] * 65535.0F + 0.5F);
-                reinterpret_cast<std::uint16_t*>(row)[3 * x + k] = static_cast<std::uint16_t>(v);
-                q[k] = static_cast<int>((v * static_cast<unsigned int>(bins - 1) + 32767U) / 65535U);
+              for (int w = 0; w < 3; ++w) {
+                o.v[w] = byte[4 * w] | (byte[4 * w + 1] << 8) | (byte[4 * w + 2] << 16) | (byte[4 * w + 3] << 24);
               }
-              break;
-            default:
+              *reinterpret_cast<W3*>(orow + 3 * xo) = o;
+            } else {
 #pragma unroll
-              for (int k = 0; k < 3; ++k) {
-                reinterpret_cast<float*>(row)[3 * x + k] = t[k];
-                q[k] = static_cast<int>(clamp01(t[k]) * static_cast<float>(bins - 1) + 0.5F);
+              for (int i = 0; i < 4; ++i) {
+                if (!valid[i]) continue;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) orow[3 * (xo + i) + k] = static_cast<std::uint8_t>(byte[3 * i + k]);
               }
-              break;
+            }
+            break;
           }
-          if (want_hist) {
-            const int luma = (2126 * q[0] + 7152 * q[1] + 722 * q[2] + 5000) / 10000;
-            atomicAdd(&hist[luma], 1U);
-            atomicAdd(&hist[bins + q[0]], 1U);
-            atomicAdd(&hist[2 * bins + q[1]], 1U);
-            atomicAdd(&hist[3 * bins + q[2]], 1U);
+          case ISP_OUT_RGBA8: {
+            unsigned int word[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              word[i] = 0xFF000000U;
+#pragma unroll
+              for (int k = 0; k < 3; ++k) {
+                const unsigned int v = static_cast<unsigned char>(t[i][k] * 255.0F + 0.5F);
+                q[i][k] = static_cast<int>(v);
+                word[i] |= v << (8 * k);
+              }
+            }
+            if (whole) {
+              W4 o;
+#pragma unroll
+              for (int i = 0; i < 4; ++i) o.v[i] = word[i];
+              *reinterpret_cast<W4*>(orow + 4 * xo) = o;
+            } else {
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                if (!valid[i]) continue;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) orow[4 * (xo + i) + k] = static_cast<std::uint8_t>(word[i] >> (8 * k));
+              }
+            }
+            break;
+          }
+          case ISP_OUT_RGB16: {
+            unsigned int half[12];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+#pragma unroll
+              for (int k = 0; k < 3; ++k) {
+                const unsigned int v = static_cast<std::uint16_t>(t[i][k] * 65535.0F + 0.5F);
+                half[3 * i + k] = v;
+                q[i][k] = static_cast<int>((v * static_cast<unsigned int>(bins - 1) + 32767U) / 65535U);
+              }
+            }
+            if (whole) {
+#pragma unroll
+              for (int w = 0; w < 3; ++w) {
+                W2 o;
+                o.v[0] = half[4 * w] | (half[4 * w + 1] << 16);
+                o.v[1] = half[4 * w + 2] | (half[4 * w + 3] << 16);
+                *reinterpret_cast<W2*>(orow + 6 * xo + 8 * w) = o;
+              }
+            } else {
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                if (!valid[i]) continue;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                  const std::uint16_t v = static_cast<std::uint16_t>(half[3 * i + k]);
+                  orow[6 * (xo + i) + 2 * k] = static_cast<std::uint8_t>(v & 0xFF);
+                  orow[6 * (xo + i) + 2 * k + 1] = static_cast<std::uint8_t>(v >> 8);
+                }
+              }
+            }
+            break;
+          }
+          default: {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+#pragma unroll
+              for (int k = 0; k < 3; ++k) {
+                q[i][k] = static_cast<int>(clamp01(t[i][k]) * static_cast<float>(bins - 1) + 0.5F);
+              }
+            }
+            if (whole) {
+#pragma unroll
+              for (int w = 0; w < 3; ++w) {
+                F4 o;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) o.v[k] = t[(4 * w + k) / 3][(4 * w + k) % 3];
+                *reinterpret_cast<F4*>(orow + 12 * xo + 16 * w) = o;
+              }
+            } else {
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                if (!valid[i]) continue;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) reinterpret_cast<float*>(orow)[3 * (xo + i) + k] = t[i][k];
+              }
+            }
+            break;
+          }
+        }
+        if (want_hist) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (!valid[i]) continue;
+            const int luma = (2126 * q[i][0] + 7152 * q[i][1] + 722 * q[i][2] + 5000) / 10000;
+            if (!wide) {
+              atomicAdd(&hist[luma], 1U);
+              atomicAdd(&hist[ISP_HISTOGRAM_BINS + q[i][0]], 1U);
+              atomicAdd(&hist[2 * ISP_HISTOGRAM_BINS + q[i][1]], 1U);
+              atomicAdd(&hist[3 * ISP_HISTOGRAM_BINS + q[i][2]], 1U);
+            } else {
+              atomicAdd(&s->main.hist[0][luma], 1U);
+              atomicAdd(&s->main.hist[1][q[i][0]], 1U);
+              atomicAdd(&s->main.hist[2][q[i][1]], 1U);
+              atomicAdd(&s->main.hist[3][q[i][2]], 1U);
+            }
           }
         }
       }
@@ -552,9 +1010,9 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
     for (int w = 0; w < kThreads / 64; ++w) total += part[w][tid];
     if (total != 0) atomicAdd(&s->main.sum[tid], total);
   }
-  if (want_hist) {
-    for (int i = tid; i < 4 * bins; i += kThreads) {
-      if (hist[i] != 0U) atomicAdd(&s->main.hist[i / bins][i % bins], hist[i]);
+  if (want_hist && !wide) {
+    for (int i = tid; i < 4 * ISP_HISTOGRAM_BINS; i += kThreads) {
+      if (hist[i] != 0U) atomicAdd(&s->main.hist[i / ISP_HISTOGRAM_BINS][i % ISP_HISTOGRAM_BINS], hist[i]);
     }
   }
   if (!last_block(&s->main.done)) return;
@@ -590,41 +1048,93 @@ int blocks_per_cu(Kernel kernel) {
   const auto it = cache.find(reinterpret_cast<const void*>(kernel));
   if (it != cache.end()) return it->second;
   int blocks = 0;
-  check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kThreads, 0), "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kThreads, 0),
+        "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   blocks = std::max(blocks, 1);
   cache[reinterpret_cast<const void*>(kernel)] = blocks;
   return blocks;
 }
 
-template <bool kMalvar, bool kSharpen>
+// Rows can be read with aligned chunk loads.
+bool rows_aligned(const FrameDesc& f) {
+  const auto p = reinterpret_cast<unsigned long long>(f.raw);
+  return ((p | static_cast<unsigned long long>(f.raw_pitch)) & 15ULL) == 0;
+}
+
+bool out_aligned(const FrameDesc& f) {
+  const auto p = reinterpret_cast<unsigned long long>(f.out);
+  return ((p | static_cast<unsigned long long>(f.out_pitch)) & 15ULL) == 0;
+}
+
+template <int kFmt, bool kMalvar, bool kSharpen>
 void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream) {
-  constexpr int kTileW = 64;
-  constexpr int kTileH = 32;
-  auto kernel = main_kernel<kMalvar, kSharpen, kTileW, kTileH>;
-  const int tiles_x = (f.width + kTileW - 1) / kTileW;
-  const int tiles_y = (f.height + kTileH - 1) / kTileH;
+  auto kernel = main_kernel<kFmt, kMalvar, kSharpen>;
+  const int tiles_x = (f.width + kTile - 1) / kTile;
+  const int tiles_y = (f.height + kTile - 1) / kTile;
   const int tiles = tiles_x * tiles_y;
   const int grid = std::max(1, std::min(tiles, cfg.cu_count * blocks_per_cu(kernel)));
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, f, s, tiles_x, tiles);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, f, s, tiles_x, tiles, rows_aligned(f) ? 1 : 0,
+                     out_aligned(f) ? 1 : 0);
   check(hipGetLastError(), "main_kernel launch");
+}
+
+template <int kFmt>
+void launch_main_f(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen, const LaunchConfig& cfg,
+                   hipStream_t st) {
+  if (malvar) {
+    sharpen ? launch_main_t<kFmt, true, true>(f, s, cfg, st) : launch_main_t<kFmt, true, false>(f, s, cfg, st);
+  } else {
+    sharpen ? launch_main_t<kFmt, false, true>(f, s, cfg, st) : launch_main_t<kFmt, false, false>(f, s, cfg, st);
+  }
+}
+
+template <int kFmt>
+void launch_awb_f(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream) {
+  auto kernel = awb_kernel<kFmt>;
+  const int tiles_x = (f.width + kTile - 1) / kTile;
+  const int tiles_y = (f.height + kTile - 1) / kTile;
+  const int tiles = tiles_x * tiles_y;
+  const int grid = std::max(1, std::min(tiles, cfg.cu_count * blocks_per_cu(kernel)));
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, f, s, tiles_x, tiles, rows_aligned(f) ? 1 : 0);
+  check(hipGetLastError(), "awb_kernel launch");
 }
 
 }  // namespace
 
 void launch_awb(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, void* stream) {
-  const int rows = f.height / 2;
-  const int grid = std::max(1, std::min(rows, cfg.cu_count * 4));
-  hipLaunchKernelGGL(awb_kernel, dim3(grid), dim3(kThreads), 0, static_cast<hipStream_t>(stream), f, s);
-  check(hipGetLastError(), "awb_kernel launch");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (f.raw_format) {
+    case ISP_RAW_U8:
+      launch_awb_f<ISP_RAW_U8>(f, s, cfg, st);
+      break;
+    case ISP_RAW_U16:
+      launch_awb_f<ISP_RAW_U16>(f, s, cfg, st);
+      break;
+    case ISP_RAW_RAW10:
+      launch_awb_f<ISP_RAW_RAW10>(f, s, cfg, st);
+      break;
+    default:
+      launch_awb_f<ISP_RAW_RAW12>(f, s, cfg, st);
+      break;
+  }
 }
 
 void launch_main(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen, const LaunchConfig& cfg,
                  void* stream) {
   const hipStream_t st = static_cast<hipStream_t>(stream);
-  if (malvar) {
-    sharpen ? launch_main_t<true, true>(f, s, cfg, st) : launch_main_t<true, false>(f, s, cfg, st);
-  } else {
-    sharpen ? launch_main_t<false, true>(f, s, cfg, st) : launch_main_t<false, false>(f, s, cfg, st);
+  switch (f.raw_format) {
+    case ISP_RAW_U8:
+      launch_main_f<ISP_RAW_U8>(f, s, malvar, sharpen, cfg, st);
+      break;
+    case ISP_RAW_U16:
+      launch_main_f<ISP_RAW_U16>(f, s, malvar, sharpen, cfg, st);
+      break;
+    case ISP_RAW_RAW10:
+      launch_main_f<ISP_RAW_RAW10>(f, s, malvar, sharpen, cfg, st);
+      break;
+    default:
+      launch_main_f<ISP_RAW_RAW12>(f, s, malvar, sharpen, cfg, st);
+      break;
   }
 }
 
