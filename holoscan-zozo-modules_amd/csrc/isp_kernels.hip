@@ -166,26 +166,28 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
-__device__ __forceinline__ unsigned int atomic_load_u32(unsigned int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long atomic_load_u64(unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Whether this block is the last of the grid to get here; publishes the
-// block's earlier writes and atomics first.
-__device__ __forceinline__ bool last_block(unsigned int* done) {
+// Grid-wide completion without cache-wide fences. Blocks publish their
+// partial sums with device-scope atomics whose results each thread consumes
+// (folded into `token`), so every atomic has been performed at the point of
+// coherence before the thread reaches the barrier; only then does one thread
+// take a ticket. The last block reads the totals back with atomic exchanges,
+// which also clear them for the next frame. (An agent-scope fence here would
+// write back and invalidate the whole L2 of the block's XCD, once per block.)
+__device__ __forceinline__ bool last_block(unsigned int* done, unsigned int token) {
   __shared__ int is_last;
-  __threadfence();
+  __shared__ unsigned int sink;
+  if (token == 0xFFFFFFFFU) sink = token;  // waits for the atomics' results
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned int ticket = atomicAdd(done, 1U);
     is_last = ticket == gridDim.x - 1 ? 1 : 0;
   }
   __syncthreads();
-  if (is_last) __threadfence();
   return is_last != 0;
+}
+
+__device__ __forceinline__ unsigned int fold(unsigned long long v) {
+  return static_cast<unsigned int>(v) ^ static_cast<unsigned int>(v >> 32);
 }
 
 // ---- lens shading ---------------------------------------------------------
@@ -439,28 +441,28 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
     if ((tid & 63) == 0) part[wave][k] = v;
   }
   __syncthreads();
+  unsigned int token = 0;
   if (tid < 4) {
     unsigned long long total = 0;
     for (int w = 0; w < kThreads / 64; ++w) total += part[w][tid];
-    if (total != 0) atomicAdd(tid < 3 ? &s->awb.sum[tid] : &s->awb.quads, total);
+    if (total != 0) token ^= fold(atomicAdd(tid < 3 ? &s->awb.sum[tid] : &s->awb.quads, total));
   }
   if (patch) {
     for (int i = tid; i < 3 * kAwbBins; i += kThreads) {
-      if (hist[i] != 0U) atomicAdd(&s->awb.hist[i / kAwbBins][i % kAwbBins], hist[i]);
+      if (hist[i] != 0U) token ^= atomicAdd(&s->awb.hist[i / kAwbBins][i % kAwbBins], hist[i]);
     }
   }
-  if (!last_block(&s->awb.done)) return;
+  if (!last_block(&s->awb.done, token)) return;
 
-  // The last block: finish_white_balance (isp_math.hpp:357-384), then clear.
+  // The last block: finish_white_balance (isp_math.hpp:357-384); the reads clear.
   __shared__ float level_s[3];
-  __shared__ unsigned long long quads_s;
-  if (tid == 0) quads_s = atomic_load_u64(&s->awb.quads);
+  __shared__ unsigned long long sums_s[4];
+  if (tid < 4) sums_s[tid] = atomicExch(tid < 3 ? &s->awb.sum[tid] : &s->awb.quads, 0ULL);
   if (patch) {
-    for (int i = tid; i < 3 * kAwbBins; i += kThreads) {
-      hist[i] = atomic_load_u32(&s->awb.hist[i / kAwbBins][i % kAwbBins]);
-    }
+    for (int i = tid; i < 3 * kAwbBins; i += kThreads) hist[i] = atomicExch(&s->awb.hist[i / kAwbBins][i % kAwbBins], 0U);
   }
   __syncthreads();
+  const unsigned long long quads_s = sums_s[3];
   if (quads_s != 0) {
     if (patch) {
       if (tid < 64) {
@@ -471,9 +473,9 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
       }
     } else if (tid == 0) {
       const double quads = static_cast<double>(quads_s);
-      level_s[0] = static_cast<float>(static_cast<double>(atomic_load_u64(&s->awb.sum[0])) / quads);
-      level_s[1] = static_cast<float>(static_cast<double>(atomic_load_u64(&s->awb.sum[1])) / (2.0 * quads));
-      level_s[2] = static_cast<float>(static_cast<double>(atomic_load_u64(&s->awb.sum[2])) / quads);
+      level_s[0] = static_cast<float>(static_cast<double>(sums_s[0]) / quads);
+      level_s[1] = static_cast<float>(static_cast<double>(sums_s[1]) / (2.0 * quads));
+      level_s[2] = static_cast<float>(static_cast<double>(sums_s[2]) / quads);
     }
   }
   __syncthreads();
@@ -492,12 +494,7 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
       st.valid = 1;
     }
   }
-  if (tid < 3) s->awb.sum[tid] = 0;
-  if (tid == 3) s->awb.quads = 0;
-  if (patch) {
-    for (int i = tid; i < 3 * kAwbBins; i += kThreads) s->awb.hist[i / kAwbBins][i % kAwbBins] = 0U;
-  }
-  if (tid == 0) s->awb.done = 0U;
+  if (tid == 0) atomicExch(&s->awb.done, 0U);
 }
 
 // ---- the main pass ----------------------------------------------------------
@@ -630,6 +627,7 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
     for (int i = tid; i < 4 * ISP_HISTOGRAM_BINS; i += kThreads) hist[i] = 0U;
   }
   unsigned long long acc[4] = {0, 0, 0, 0};
+  unsigned int wide_token = 0;
   const int W = f.width;
   const int H = f.height;
   const float* __restrict__ lut = f.lut;
@@ -984,10 +982,10 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
               atomicAdd(&hist[2 * ISP_HISTOGRAM_BINS + q[i][1]], 1U);
               atomicAdd(&hist[3 * ISP_HISTOGRAM_BINS + q[i][2]], 1U);
             } else {
-              atomicAdd(&s->main.hist[0][luma], 1U);
-              atomicAdd(&s->main.hist[1][q[i][0]], 1U);
-              atomicAdd(&s->main.hist[2][q[i][1]], 1U);
-              atomicAdd(&s->main.hist[3][q[i][2]], 1U);
+              wide_token ^= atomicAdd(&s->main.hist[0][luma], 1U);
+              wide_token ^= atomicAdd(&s->main.hist[1][q[i][0]], 1U);
+              wide_token ^= atomicAdd(&s->main.hist[2][q[i][1]], 1U);
+              wide_token ^= atomicAdd(&s->main.hist[3][q[i][2]], 1U);
             }
           }
         }
@@ -1005,33 +1003,31 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
     }
   }
   __syncthreads();
+  unsigned int token = wide_token;
   if (want_stats && tid < 4) {
     unsigned long long total = 0;
     for (int w = 0; w < kThreads / 64; ++w) total += part[w][tid];
-    if (total != 0) atomicAdd(&s->main.sum[tid], total);
+    if (total != 0) token ^= fold(atomicAdd(&s->main.sum[tid], total));
   }
   if (want_hist && !wide) {
     for (int i = tid; i < 4 * ISP_HISTOGRAM_BINS; i += kThreads) {
-      if (hist[i] != 0U) atomicAdd(&s->main.hist[i / ISP_HISTOGRAM_BINS][i % ISP_HISTOGRAM_BINS], hist[i]);
+      if (hist[i] != 0U) token ^= atomicAdd(&s->main.hist[i / ISP_HISTOGRAM_BINS][i % ISP_HISTOGRAM_BINS], hist[i]);
     }
   }
-  if (!last_block(&s->main.done)) return;
+  if (!last_block(&s->main.done, token)) return;
 
-  // The last block: the histograms and the statistics vector, then clear.
+  // The last block: the histograms and the statistics vector; the reads clear.
   if (want_hist) {
-    for (int i = tid; i < 4 * bins; i += kThreads) {
-      unsigned int* cell = &s->main.hist[i / bins][i % bins];
-      f.histogram[i] = atomic_load_u32(cell);
-      *cell = 0U;
-    }
+    for (int i = tid; i < 4 * bins; i += kThreads) f.histogram[i] = atomicExch(&s->main.hist[i / bins][i % bins], 0U);
   }
+  __shared__ unsigned long long sums_s[4];
+  if (tid < 4) sums_s[tid] = atomicExch(&s->main.sum[tid], 0ULL);
+  __syncthreads();
   if (want_stats && tid == 0) {
-    unsigned long long sums[4];
-    for (int k = 0; k < 4; ++k) sums[k] = atomic_load_u64(&s->main.sum[k]);
+    const unsigned long long sums[4] = {sums_s[0], sums_s[1], sums_s[2], sums_s[3]};
     finish_stats(sums, static_cast<long long>(W) * H, gains, f.ae, f.exposure_ns, f.exposure_gain, f.stats);
   }
-  if (tid < 4) s->main.sum[tid] = 0;
-  if (tid == 0) s->main.done = 0U;
+  if (tid == 0) atomicExch(&s->main.done, 0U);
 }
 
 void check(hipError_t status, const char* what) {
