@@ -175,8 +175,9 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // write back and invalidate the whole L2 of the block's XCD, once per block.)
 __device__ __forceinline__ bool last_block(unsigned int* done, unsigned int token) {
   __shared__ int is_last;
-  __shared__ unsigned int sink;
-  if (token == 0xFFFFFFFFU) sink = token;  // waits for the atomics' results
+  // Consuming the token makes the atomics return their values and the wave wait
+  // for them here (the asm reads a register and touches no memory).
+  __asm__ volatile("" : : "v"(token));
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned int ticket = atomicAdd(done, 1U);
