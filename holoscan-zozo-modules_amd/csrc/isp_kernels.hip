@@ -4,31 +4,33 @@
 //
 //  awb_kernel   white balance statistics (reference: white_balance_quad,
 //               isp_math.hpp:311-332; awb_kernel, cuda_pipeline.cu:119-158).
-//               Persistent blocks walk 64x64 tiles; each thread takes 4
+//               Persistent blocks walk 64x64 tiles; each thread takes kC/2
 //               complete 2x2 quads from two coalesced row-chunk loads, skips
 //               quads with a saturated sample, and sums the linearised values
 //               (2^-20 units) per colour in registers; white patch adds an
 //               LDS-privatised 3 x 1024 histogram. The last block to finish
 //               turns the sums into gains (finish_white_balance,
-//               isp_math.hpp:357-384) in the pipeline's device state and clears
-//               the sums for the next frame.
+//               isp_math.hpp:357-384) in the pipeline's device state.
 //
-//  main_kernel  persistent blocks walk 64x64 output tiles:
-//               1. coalesced chunk loads of the raw rows (RAW12: 8 samples from
-//                  3 dwords, RAW10: 16 from 5, u16: 8 from 4, u8: 8 from 2),
-//                  unpack, black level, lens shading, white balance and clip
-//                  into an LDS tile with a mirrored halo;
+//  main_kernel  persistent blocks walk 64 x 32 output tiles:
+//               1. every thread first issues its coalesced row-chunk loads
+//                  (RAW12: 8 samples from 3 dwords, RAW10: 16 from 5, u16: 8
+//                  from 4, u8: 8 from 2), then unpacks them and applies black
+//                  level, lens shading, white balance and the clip, writing
+//                  four samples per 128-bit LDS store into a tile with a
+//                  mirrored halo;
 //               2. with sharpening, Malvar / bilinear demosaic of the tile and
-//                  a one-pixel ring into LDS (4x2 pixels per thread, windows
+//                  a one-pixel ring into LDS (8 x 2 pixels per thread, windows
 //                  read as 128-bit LDS loads, only the two formulas each CFA
 //                  site needs);
-//               3. per thread a 4x4 block of output pixels: unsharp mask (or
-//                  the demosaic itself), colour matrix, tone table, one vector
-//                  store per row, the statistics sums in registers and the
-//                  4 x bins histograms of the written codes in LDS.
+//               3. per thread a 4 x 2 block of output pixels: unsharp mask (or
+//                  the demosaic itself), colour matrix, tone table (from LDS),
+//                  one vector store per row, the statistics sums in registers
+//                  and the 4 x bins histograms of the written codes in four
+//                  lane-interleaved LDS copies (fewer same-address conflicts).
 //               The last block writes the histograms and the statistics vector
 //               (finish_stats, isp_math.hpp:387-425, with the exact log2/exp2
-//               of isp_ae.hpp) and clears the scratch.
+//               of isp_ae.hpp).
 //
 // Lens shading: the reference's per-sample bilinear interpolation
 // (lens_shading_gain, isp_math.hpp:121-140) is top + (bottom - top) * ty with
@@ -54,7 +56,6 @@ namespace isp_amd {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kTile = 64;  // output tile edge (main and white balance passes)
 
 struct alignas(4) W3 {
   unsigned int v[3];
@@ -125,41 +126,6 @@ __device__ __forceinline__ unsigned int byte_of(const unsigned int* w, int i) {
   return (w[i >> 2] >> (8 * (i & 3))) & 0xFFU;
 }
 
-// kSamples samples at x (a multiple of kSamples) of an aligned row.
-template <int kFmt>
-__device__ __forceinline__ void load_chunk(const std::uint8_t* row, int x, int* s) {
-  if constexpr (kFmt == ISP_RAW_U8) {
-    const W2 w = *reinterpret_cast<const W2*>(row + x);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) s[k] = static_cast<int>(byte_of(w.v, k));
-  } else if constexpr (kFmt == ISP_RAW_U16) {
-    const W4 w = *reinterpret_cast<const W4*>(row + 2 * x);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      s[2 * k] = static_cast<int>(w.v[k] & 0xFFFFU);
-      s[2 * k + 1] = static_cast<int>(w.v[k] >> 16);
-    }
-  } else if constexpr (kFmt == ISP_RAW_RAW12) {
-    const W3 w = *reinterpret_cast<const W3*>(row + 3 * (x >> 1));
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const unsigned int lo = byte_of(w.v, 3 * g + 2);
-      s[2 * g] = static_cast<int>((byte_of(w.v, 3 * g) << 4) | (lo & 15U));
-      s[2 * g + 1] = static_cast<int>((byte_of(w.v, 3 * g + 1) << 4) | (lo >> 4));
-    }
-  } else {
-    const W5 w = *reinterpret_cast<const W5*>(row + 5 * (x >> 2));
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const unsigned int lo = byte_of(w.v, 5 * g + 4);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        s[4 * g + k] = static_cast<int>((byte_of(w.v, 5 * g + k) << 2) | ((lo >> (2 * k)) & 3U));
-      }
-    }
-  }
-}
-
 // Wave64 sum of a 64-bit value.
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset, 64);
@@ -193,6 +159,8 @@ __device__ __forceinline__ unsigned int fold(unsigned long long v) {
 
 // ---- lens shading ---------------------------------------------------------
 
+constexpr int kMaxGrid = 1024;  // lens shading grid values staged in LDS when they fit
+
 // lens_shading_gain (isp_math.hpp:121-140), per sample: the slow path for
 // tiles whose rows span more than two grid rows.
 __device__ __forceinline__ float lsc_gain(const FrameDesc& f, int x, int y, int channel) {
@@ -212,6 +180,7 @@ __device__ __forceinline__ float lsc_gain(const FrameDesc& f, int x, int y, int 
   return top + (bottom - top) * ty;
 }
 
+// The grid row of frame row y and the weight ty within it.
 __device__ __forceinline__ int lsc_row(const FrameDesc& f, int y, float* ty) {
   const float fy = static_cast<float>(y) * f.lsc_sy;
   int iy = static_cast<int>(fy);
@@ -219,16 +188,22 @@ __device__ __forceinline__ int lsc_row(const FrameDesc& f, int y, float* ty) {
   *ty = fy - static_cast<float>(iy);
   return iy;
 }
+__device__ __forceinline__ int lsc_row(const FrameDesc& f, int y) {
+  float ty;
+  return lsc_row(f, y, &ty);
+}
 
-// The per-tile lens shading table: for sample columns cx0 .. cx0 + kCols - 1,
-// both row parities and grid rows iy0, iy0 + 1: (top, bottom - top).
+// Per-tile table for sample columns cx0 .. cx0 + kCols - 1, both row parities
+// and grid rows iy0, iy0 + 1: (top, bottom - top) of lens_shading_gain.
 template <int kCols>
 struct LscTable {
   F2 e[2][2][kCols];  // [grid row - iy0][row parity][column]
 };
 
+// `grid` is the lens shading grid (LDS or global memory).
 template <int kCols>
-__device__ void build_lsc_table(const FrameDesc& f, int cx0, int iy0, LscTable<kCols>& t) {
+__device__ __forceinline__ void build_lsc_table(const FrameDesc& f, const float* grid, int cx0, int iy0,
+                                                LscTable<kCols>& t) {
   const int step = f.lsc_channels;
   const int row = f.lsc_width * step;
   for (int i = threadIdx.x; i < 4 * kCols; i += blockDim.x) {
@@ -243,7 +218,7 @@ __device__ void build_lsc_table(const FrameDesc& f, int cx0, int iy0, LscTable<k
     ix = ix > f.lsc_width - 2 ? f.lsc_width - 2 : ix;
     const float tx = fx - static_cast<float>(ix);
     const int channel = cfa_channel(f.cfa, (par << 1) | (x & 1));
-    const float* g = f.lsc + (iy * f.lsc_width + ix) * step + (step == 4 ? channel : 0);
+    const float* g = grid + (iy * f.lsc_width + ix) * step + (step == 4 ? channel : 0);
     const float top = g[0] + (g[step] - g[0]) * tx;
     const float bottom = g[row] + (g[row + step] - g[row]) * tx;
     t.e[seg][par][c].v[0] = top;
@@ -251,24 +226,13 @@ __device__ void build_lsc_table(const FrameDesc& f, int cx0, int iy0, LscTable<k
   }
 }
 
-// The range of grid rows the tile's (mirrored) sample rows [r0, r0 + n) fall in.
-__device__ void lsc_row_range(const FrameDesc& f, int r0, int n, int* lo, int* hi) {
-  __shared__ int s_lo;
-  __shared__ int s_hi;
-  if (threadIdx.x == 0) {
-    s_lo = 1 << 30;
-    s_hi = -1;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    float ty;
-    const int iy = lsc_row(f, mirror_near(r0 + i, f.height), &ty);
-    atomicMin(&s_lo, iy);
-    atomicMax(&s_hi, iy);
-  }
-  __syncthreads();
-  *lo = s_lo;
-  *hi = s_hi;
+// Copies the lens shading grid into LDS when it fits; returns where to read it.
+__device__ __forceinline__ const float* stage_grid(const FrameDesc& f, float* grid_s) {
+  if (f.lsc == nullptr) return nullptr;
+  const int n = f.lsc_width * f.lsc_height * f.lsc_channels;
+  if (n > kMaxGrid) return f.lsc;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) grid_s[i] = f.lsc[i];
+  return grid_s;
 }
 
 // Per quad position p = (y & 1) * 2 + (x & 1): black, 1 / range, colour gain.
@@ -288,6 +252,64 @@ __device__ __forceinline__ QuadConst quad_consts(const FrameDesc& f, const float
     q.gain[p] = gains[color_of(ch)];
   }
   return q;
+}
+
+// Raw words of one chunk (kSamples samples), as loaded.
+template <int kFmt>
+struct Packed {
+  static constexpr int kWords = kFmt == ISP_RAW_U8 ? 2 : kFmt == ISP_RAW_U16 ? 4 : kFmt == ISP_RAW_RAW12 ? 3 : 5;
+  unsigned int w[kWords];
+};
+
+template <int kFmt>
+__device__ __forceinline__ void load_packed(const std::uint8_t* row, int x, Packed<kFmt>& p) {
+  if constexpr (kFmt == ISP_RAW_U8) {
+    const W2 w = *reinterpret_cast<const W2*>(row + x);
+    p.w[0] = w.v[0];
+    p.w[1] = w.v[1];
+  } else if constexpr (kFmt == ISP_RAW_U16) {
+    const W4 w = *reinterpret_cast<const W4*>(row + 2 * x);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p.w[k] = w.v[k];
+  } else if constexpr (kFmt == ISP_RAW_RAW12) {
+    const W3 w = *reinterpret_cast<const W3*>(row + 3 * (x >> 1));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p.w[k] = w.v[k];
+  } else {
+    const W5 w = *reinterpret_cast<const W5*>(row + 5 * (x >> 2));
+#pragma unroll
+    for (int k = 0; k < 5; ++k) p.w[k] = w.v[k];
+  }
+}
+
+template <int kFmt>
+__device__ __forceinline__ void unpack(const Packed<kFmt>& p, int* s) {
+  if constexpr (kFmt == ISP_RAW_U8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = static_cast<int>(byte_of(p.w, k));
+  } else if constexpr (kFmt == ISP_RAW_U16) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      s[2 * k] = static_cast<int>(p.w[k] & 0xFFFFU);
+      s[2 * k + 1] = static_cast<int>(p.w[k] >> 16);
+    }
+  } else if constexpr (kFmt == ISP_RAW_RAW12) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const unsigned int lo = byte_of(p.w, 3 * g + 2);
+      s[2 * g] = static_cast<int>((byte_of(p.w, 3 * g) << 4) | (lo & 15U));
+      s[2 * g + 1] = static_cast<int>((byte_of(p.w, 3 * g + 1) << 4) | (lo >> 4));
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const unsigned int lo = byte_of(p.w, 5 * g + 4);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        s[4 * g + k] = static_cast<int>((byte_of(p.w, 5 * g + k) << 2) | ((lo >> (2 * k)) & 3U));
+      }
+    }
+  }
 }
 
 // ---- white balance statistics -------------------------------------------
@@ -325,19 +347,26 @@ __device__ float percentile_wave(const unsigned int* hist, float percentile) {
   return static_cast<float>((static_cast<double>(bin) + w) / kAwbBins);
 }
 
+constexpr int kAwbTile = 64;
+
 template <int kFmt>
 __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles,
                                                        int fast_rows) {
   constexpr int kC = Chunk<kFmt>::kSamples;
-  constexpr int kCols = kTile;
+  constexpr int kCols = kAwbTile;
+  constexpr int kTasksX = kAwbTile / kC;
+  constexpr int kTasks = kTasksX * (kAwbTile / 2);
+  static_assert(kTasks <= kThreads, "one task per thread");
   __shared__ unsigned int hist[3 * kAwbBins];
   __shared__ unsigned long long part[kThreads / 64][4];
-  __shared__ LscTable<kCols> lsc_tab;
+  __shared__ __attribute__((aligned(16))) LscTable<kCols> lsc_tab;
+  __shared__ float grid_s[kMaxGrid];
   const bool patch = f.wb_mode == ISP_WB_WHITE_PATCH;
   const int tid = threadIdx.x;
   if (patch) {
     for (int i = tid; i < 3 * kAwbBins; i += kThreads) hist[i] = 0U;
   }
+  const float* grid = stage_grid(f, grid_s);
   QuadConst qc;
   {
     const float ones[3] = {1.0F, 1.0F, 1.0F};
@@ -346,94 +375,98 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
   int colour[4];
 #pragma unroll
   for (int p = 0; p < 4; ++p) colour[p] = color_of(cfa_channel(f.cfa, p));
-  const int qw = f.width / 2;
-  const int qh = f.height / 2;
+  const int W = f.width;
+  const int H = f.height;
+  const int qw = W / 2;
+  const int qh = H / 2;
+  const bool lsc = f.lsc != nullptr;
   unsigned long long local[4] = {0, 0, 0, 0};
 
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int tyi = tile / tiles_x;
-    const int x0 = (tile - tyi * tiles_x) * kTile;
-    const int y0 = tyi * kTile;
+    const int x0 = (tile - tyi * tiles_x) * kAwbTile;
+    const int y0 = tyi * kAwbTile;
+    const int qr = tid / kTasksX;
+    const int cx = x0 + (tid - qr * kTasksX) * kC;
+    const int qy = (y0 >> 1) + qr;
+    const bool active = tid < kTasks && qy < qh && cx < 2 * qw;
+    const bool whole = active && fast_rows != 0 && cx + kC <= W;
+    const std::uint8_t* row0 = f.raw + static_cast<long long>(2 * qy) * f.raw_pitch;
+    const std::uint8_t* row1 = row0 + f.raw_pitch;
+    Packed<kFmt> pk[2];
+    if (whole) {
+      load_packed<kFmt>(row0, cx, pk[0]);
+      load_packed<kFmt>(row1, cx, pk[1]);
+    }
+    int lo = 0;
     bool table = false;
-    int iy0 = 0;
     __syncthreads();  // the previous tile's table readers are done
-    if (f.lsc != nullptr) {
-      int lo, hi;
-      lsc_row_range(f, y0, min(kTile, f.height - y0), &lo, &hi);
-      table = hi - lo <= 1;
-      iy0 = lo;
-      if (table) build_lsc_table<kCols>(f, x0, iy0, lsc_tab);
-      __syncthreads();
+    if (lsc) {
+      lo = lsc_row(f, y0);
+      table = lsc_row(f, min(y0 + kAwbTile - 1, H - 1)) - lo <= 1;
+      if (table) build_lsc_table<kCols>(f, grid, x0, lo, lsc_tab);
     }
-    // Each task: kC columns x 2 rows = kC / 2 quads.
-    constexpr int kTasksX = kTile / kC;
-    constexpr int kTasks = kTasksX * (kTile / 2);
-    for (int task = tid; task < kTasks; task += kThreads) {
-      const int qr = task / kTasksX;
-      const int cx = x0 + (task - qr * kTasksX) * kC;
-      const int qy = (y0 >> 1) + qr;
-      if (qy >= qh || cx >= 2 * qw) continue;
-      int raw[2][kC];
-      const std::uint8_t* rows[2] = {f.raw + static_cast<long long>(2 * qy) * f.raw_pitch,
-                                     f.raw + static_cast<long long>(2 * qy + 1) * f.raw_pitch};
-      const bool whole = fast_rows != 0 && cx + kC <= f.width;
+    __syncthreads();
+    if (!active) continue;
+    int raw[2][kC];
+    if (whole) {
+      unpack<kFmt>(pk[0], raw[0]);
+      unpack<kFmt>(pk[1], raw[1]);
+    } else {
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        if (whole) {
-          load_chunk<kFmt>(rows[r], cx, raw[r]);
-        } else {
+      for (int k = 0; k < kC; ++k) {
+        raw[0][k] = cx + k < 2 * qw ? load_raw(row0, kFmt, cx + k) : 0;
+        raw[1][k] = cx + k < 2 * qw ? load_raw(row1, kFmt, cx + k) : 0;
+      }
+    }
+    float tyr[2] = {0.0F, 0.0F};
+    int seg[2] = {0, 0};
+    if (table) {
 #pragma unroll
-          for (int k = 0; k < kC; ++k) raw[r][k] = cx + k < 2 * qw ? load_raw(rows[r], kFmt, cx + k) : 0;
+      for (int r = 0; r < 2; ++r) seg[r] = lsc_row(f, 2 * qy + r, &tyr[r]) - lo;
+    }
+    unsigned int sum[3] = {0U, 0U, 0U};
+    unsigned int quads = 0U;
+#pragma unroll
+    for (int q = 0; q < kC / 2; ++q) {
+      if ((cx >> 1) + q >= qw) continue;
+      bool ok = true;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) ok = ok && raw[k >> 1][2 * q + (k & 1)] < f.white;
+      if (!ok) continue;
+      quads += 1U;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int r = k >> 1;
+        const int c = 2 * q + (k & 1);
+        // white_balance_quad (isp_math.hpp:311-332): linearize, cap at 4, fixed point, bin
+        float v = (static_cast<float>(raw[r][c]) - qc.black[k]) * qc.inv[k];
+        v = v > 0.0F ? v : 0.0F;
+        if (lsc) {
+          float gain;
+          if (table) {
+            const F2 e = lsc_tab.e[seg[r]][r][cx - x0 + c];
+            gain = e.v[0] + e.v[1] * tyr[r];
+          } else {
+            gain = lsc_gain(f, cx + c, 2 * qy + r, cfa_channel(f.cfa, k));
+          }
+          v = v * gain;
+        }
+        v = v < 4.0F ? v : 4.0F;
+        const int col = colour[k];
+        const unsigned int fixed = static_cast<unsigned int>(v * 1048576.0F + 0.5F);
+        sum[0] += col == 0 ? fixed : 0U;
+        sum[1] += col == 1 ? fixed : 0U;
+        sum[2] += col == 2 ? fixed : 0U;
+        if (patch) {
+          const int b = static_cast<int>(v * static_cast<float>(kAwbBins));
+          atomicAdd(&hist[col * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
         }
       }
-      float tyr[2] = {0.0F, 0.0F};
-      int seg[2] = {0, 0};
-      if (table) {
-#pragma unroll
-        for (int r = 0; r < 2; ++r) seg[r] = lsc_row(f, 2 * qy + r, &tyr[r]) - iy0;
-      }
-      unsigned int sum[3] = {0U, 0U, 0U};
-      unsigned int quads = 0U;
-#pragma unroll
-      for (int q = 0; q < kC / 2; ++q) {
-        const int qx = (cx >> 1) + q;
-        if (qx >= qw) continue;
-        bool ok = true;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) ok = ok && raw[k >> 1][2 * q + (k & 1)] < f.white;
-        if (!ok) continue;
-        quads += 1U;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int r = k >> 1;
-          const int c = 2 * q + (k & 1);
-          const int p = k;  // quad position: (y & 1) * 2 + (x & 1); quads start at even (x, y)
-          float v = (static_cast<float>(raw[r][c]) - qc.black[p]) * qc.inv[p];
-          v = v > 0.0F ? v : 0.0F;
-          if (f.lsc != nullptr) {
-            float gain;
-            if (table) {
-              const F2 e = lsc_tab.e[seg[r]][r][cx - x0 + c];
-              gain = e.v[0] + e.v[1] * tyr[r];
-            } else {
-              gain = lsc_gain(f, cx + c, 2 * qy + r, cfa_channel(f.cfa, p));
-            }
-            v = v * gain;
-          }
-          v = v < 4.0F ? v : 4.0F;
-          const int col = colour[p];
-          const unsigned int fixed = static_cast<unsigned int>(v * 1048576.0F + 0.5F);
-          sum[col] += fixed;
-          if (patch) {
-            const int b = static_cast<int>(v * static_cast<float>(kAwbBins));
-            atomicAdd(&hist[col * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) local[k] += sum[k];
-      local[3] += quads;
     }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) local[k] += sum[k];
+    local[3] += quads;
   }
   const int wave = tid >> 6;
 #pragma unroll
@@ -565,56 +598,81 @@ __device__ __forceinline__ void assign_rgb(int ch, float c, float first, float s
   }
 }
 
-// tone (isp_math.hpp:235-243) on a table in global memory.
-__device__ __forceinline__ float tone(const float* __restrict__ lut, int size, float x) {
-  const float f = x * static_cast<float>(size - 1);
+// tone (isp_math.hpp:235-243). f - (float)(int)f equals fract(f) exactly for
+// f >= 0; the index is kept in the table for the top entry, which the select
+// then replaces by lut[n1] as the reference returns it.
+__device__ __forceinline__ float tone(const float* lut, int n1, float top, float x) {
+  const float f = x * static_cast<float>(n1);
   const int i = static_cast<int>(f);
-  if (i >= size - 1) return lut[size - 1];
-  const float t = f - static_cast<float>(i);
-  const float a = lut[i];
-  return a + t * (lut[i + 1] - a);
+  const float t = __builtin_amdgcn_fractf(f);
+  const unsigned int ii = static_cast<unsigned int>(i < n1 - 1 ? i : n1 - 1);
+  const float a = lut[ii];
+  const float b = lut[ii + 1U];
+  const float v = a + t * (b - a);
+  return i >= n1 ? top : v;
 }
 
-template <int kFmt, bool kMalvar, bool kSharpen>
+constexpr int kLdsLut = 4100;  // tone tables up to this size are read from LDS
+
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
 struct MainGeom {
-  static constexpr int kRadius = kMalvar ? 2 : 1;
-  static constexpr int kHalo = kRadius + (kSharpen ? 1 : 0);
+  static constexpr int kR = kMalvar ? 2 : 1;
+  static constexpr int kH = kR + (kSharpen ? 1 : 0);
   static constexpr int kC = Chunk<kFmt>::kSamples;
-  // Sample tile: frame columns x0 - kHalo .. x0 + kTile + kHalo - 1 from LDS
-  // column 0; the stride leaves room for the 8-wide windows of the last blocks
-  // and keeps every window read 16-byte aligned.
-  static constexpr int kSW = (kTile + 2 * kHalo + 2 + 3) / 4 * 4;
-  static constexpr int kSH = kTile + 2 * kHalo;
-  // Ring of demosaiced RGB (tile plus one pixel), 4-wide blocks.
-  static constexpr int kRW = kSharpen ? (kTile + 2 + 3) / 4 * 4 : 4;
-  static constexpr int kRH = kSharpen ? kTile + 2 : 2;
-  // Lens shading table columns: the chunk-aligned span covering the sample columns.
-  static constexpr int kLscCols = (kTile + 2 * kHalo + 2 * kC - 2) / kC * kC;
+  // Sample tile: LDS column = frame x - x0 + kPad, columns x0 - 4 .. x0 + 71,
+  // rows y0 - kH .. y0 + kTileH + kH - 1; every 4-column group is 16-byte aligned.
+  static constexpr int kPad = 4;
+  static constexpr int kSW = 76;
+  static constexpr int kSH = kTileH + 2 * kH;
+  // Row chunks from x0 - kC cover the sample columns.
+  static constexpr int kNC = (kSW - kPad + kC + kC - 1) / kC;
+  static constexpr int kTasks = kSH * kNC;
+  static constexpr int kTPT = (kTasks + kThreads - 1) / kThreads;
+  // Ring of demosaiced RGB: ring column rc = frame x - x0 + 2, row rr = y - y0 + 1;
+  // 9 blocks of 8 columns, one block row of 2 per thread.
+  static constexpr int kRW = 72;
+  static constexpr int kRH = kSharpen ? kTileH + 2 : 2;
+  static constexpr int kRingBlocks = (kRW / 8) * (kRH / 2);
+  // Output rows per thread (a 4 x kRows3 block).
+  static constexpr int kRows3 = kTileH / 16;
+  static constexpr int kLscCols = kNC * kC;
+  static_assert(kRingBlocks <= kThreads || !kSharpen, "one ring block per thread");
 };
 
-template <int kFmt, bool kMalvar, bool kSharpen>
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
 __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles,
                                                         int fast_rows, int out_aligned) {
-  using G = MainGeom<kFmt, kMalvar, kSharpen>;
-  constexpr int kR = G::kRadius;
-  constexpr int kH = G::kHalo;
+  using G = MainGeom<kFmt, kMalvar, kSharpen, kTileH>;
+  constexpr int kR = G::kR;
+  constexpr int kH = G::kH;
   constexpr int kC = G::kC;
+  constexpr int kPad = G::kPad;
   constexpr int kSW = G::kSW;
   constexpr int kSH = G::kSH;
+  constexpr int kNC = G::kNC;
+  constexpr int kTasks = G::kTasks;
+  constexpr int kTPT = G::kTPT;
   constexpr int kRW = G::kRW;
   constexpr int kRH = G::kRH;
+  constexpr int kRows3 = G::kRows3;
   constexpr int kLscCols = G::kLscCols;
   __shared__ __attribute__((aligned(16))) float samp[kSH * kSW];
   __shared__ __attribute__((aligned(16))) float ring[3][kRH * kRW];
-  __shared__ unsigned int hist[4 * ISP_HISTOGRAM_BINS];
-  __shared__ LscTable<kLscCols> lsc_tab;
+  __shared__ float lut_s[kLdsLut];
+  __shared__ unsigned int hist[4 * kMaxHistBins];
+  __shared__ __attribute__((aligned(16))) LscTable<kLscCols> lsc_tab;
+  __shared__ float grid_s[kMaxGrid];
   __shared__ unsigned long long part[kThreads / 64][4];
 
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const bool want_hist = f.histogram != nullptr;
   const bool want_stats = f.stats != nullptr;
   const int bins = f.bins;
-  const bool wide = bins != ISP_HISTOGRAM_BINS;  // 1024-bin extension: global atomics
+  const int copies = kMaxHistBins / bins;  // interleaved histogram copies: 4 for 256 bins
+  const int copy = lane & (copies - 1);
+  const int W = f.width;
+  const int H = f.height;
   float gains[3];
   if (f.wb_mode != ISP_WB_MANUAL) {
 #pragma unroll
@@ -624,50 +682,67 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
     for (int k = 0; k < 3; ++k) gains[k] = f.wb[k];
   }
   const QuadConst qc = quad_consts(f, gains);
-  if (want_hist && !wide) {
-    for (int i = tid; i < 4 * ISP_HISTOGRAM_BINS; i += kThreads) hist[i] = 0U;
+  if (want_hist) {
+    for (int i = tid; i < 4 * kMaxHistBins; i += kThreads) hist[i] = 0U;
   }
+  const bool lut_lds = f.lut_size <= kLdsLut;
+  if (lut_lds) {
+    for (int i = tid; i < f.lut_size; i += kThreads) lut_s[i] = f.lut[i];
+  }
+  const int n1 = f.lut_size - 1;
+  const float lut_top = f.lut[n1];
+  const bool lsc = f.lsc != nullptr;
+  const float* grid = stage_grid(f, grid_s);
   unsigned long long acc[4] = {0, 0, 0, 0};
-  unsigned int wide_token = 0;
-  const int W = f.width;
-  const int H = f.height;
-  const float* __restrict__ lut = f.lut;
-  const int lut_size = f.lut_size;
-  const float ccm[9] = {f.ccm[0], f.ccm[1], f.ccm[2], f.ccm[3], f.ccm[4], f.ccm[5], f.ccm[6], f.ccm[7], f.ccm[8]};
 
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int tyi = tile / tiles_x;
-    const int x0 = (tile - tyi * tiles_x) * kTile;
-    const int y0 = tyi * kTile;
-    const int sx0 = x0 - kH;  // frame column of sample column 0
-    const int sy0 = y0 - kH;
-    const int cx0 = (sx0 >= 0 ? sx0 / kC : -((-sx0 + kC - 1) / kC)) * kC;  // chunk-aligned start
-    const bool edge_x = sx0 < 0 || x0 + kTile + kH > W;
+    const int x0 = (tile - tyi * tiles_x) * 64;
+    const int y0 = tyi * kTileH;
+    const int sy0 = y0 - kH;  // frame row of sample row 0
+    const int cx0 = x0 - kC;  // frame column of chunk 0
+    const bool interior_x = cx0 >= 0 && cx0 + kNC * kC <= W;
+
+    // 1a. Issue this thread's raw chunk loads.
+    Packed<kFmt> pk[kTPT];
+#pragma unroll
+    for (int t = 0; t < kTPT; ++t) {
+      const int task = tid + t * kThreads;
+      if (task >= kTasks) break;
+      const int sr = task / kNC;
+      const int c = task - sr * kNC;
+      const int cx = cx0 + c * kC;
+      if (fast_rows != 0 && cx >= 0 && cx + kC <= W) {
+        const int y = mirror_near(sy0 + sr, H);
+        load_packed<kFmt>(f.raw + static_cast<long long>(y) * f.raw_pitch, cx, pk[t]);
+      }
+    }
     __syncthreads();  // the previous tile's readers are done
+    int lo = 0;
     bool table = false;
-    int iy0 = 0;
-    if (f.lsc != nullptr) {
-      int lo, hi;
-      lsc_row_range(f, sy0, kSH, &lo, &hi);
-      table = hi - lo <= 1;
-      iy0 = lo;
-      if (table) build_lsc_table<kLscCols>(f, cx0, iy0, lsc_tab);
+    if (lsc) {
+      lo = lsc_row(f, sy0 > 0 ? sy0 : 0);
+      table = lsc_row(f, sy0 + kSH - 1 < H - 1 ? sy0 + kSH - 1 : H - 1) - lo <= 1;
+      if (table) build_lsc_table<kLscCols>(f, grid, cx0, lo, lsc_tab);
       __syncthreads();
     }
 
-    // 1. Samples (preprocess, isp_math.hpp:151-155).
+    // 1b. Unpack and preprocess (isp_math.hpp:151-155) into the sample tile.
     unsigned int saturated = 0;
-    const int n_chunks = (x0 + kTile + kH - cx0 + kC - 1) / kC;
-    for (int task = tid; task < kSH * n_chunks; task += kThreads) {
-      const int sr = task / n_chunks;
-      const int cx = cx0 + (task - sr * n_chunks) * kC;
-      const int yv = sy0 + sr;  // virtual row
+#pragma unroll
+    for (int t = 0; t < kTPT; ++t) {
+      const int task = tid + t * kThreads;
+      if (task >= kTasks) break;
+      const int sr = task / kNC;
+      const int c = task - sr * kNC;
+      const int cx = cx0 + c * kC;
+      const int yv = sy0 + sr;
       const int y = mirror_near(yv, H);
-      const std::uint8_t* row = f.raw + static_cast<long long>(y) * f.raw_pitch;
       int raw[kC];
       if (fast_rows != 0 && cx >= 0 && cx + kC <= W) {
-        load_chunk<kFmt>(row, cx, raw);
+        unpack<kFmt>(pk[t], raw);
       } else {
+        const std::uint8_t* row = f.raw + static_cast<long long>(y) * f.raw_pitch;
 #pragma unroll
         for (int k = 0; k < kC; ++k) raw[k] = cx + k >= 0 && cx + k < W ? load_raw(row, kFmt, cx + k) : 0;
       }
@@ -680,182 +755,198 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
       const float g1 = par ? qc.gain[3] : qc.gain[1];
       float tyr = 0.0F;
       int seg = 0;
-      if (table) seg = lsc_row(f, y, &tyr) - iy0;
-      const bool count = want_stats && yv >= y0 && yv < y0 + kTile && yv < H;
+      if (table) seg = lsc_row(f, y, &tyr) - lo;
+      const bool count = want_stats && yv >= y0 && yv < y0 + kTileH && yv < H && c >= 1 && c <= 64 / kC;
 #pragma unroll
-      for (int k = 0; k < kC; ++k) {
-        const int x = cx + k;
-        const int col = x - sx0;
-        if (col < 0 || col >= kSW || x < 0 || x >= W) continue;
-        const bool odd = (k & 1) != 0;  // cx is even
-        float v = (static_cast<float>(raw[k]) - (odd ? b1 : b0)) * (odd ? i1 : i0);
-        v = v > 0.0F ? v : 0.0F;
-        if (f.lsc != nullptr) {
-          float gain;
-          if (table) {
-            const F2 e = lsc_tab.e[seg][par][x - cx0];
-            gain = e.v[0] + e.v[1] * tyr;
-          } else {
-            gain = lsc_gain(f, x, y, cfa_channel(f.cfa, (par << 1) | (x & 1)));
+      for (int h = 0; h < kC / 4; ++h) {
+        const int col0 = kPad - kC + c * kC + 4 * h;
+        if (col0 < 0 || col0 + 4 > kSW) continue;
+        F4 o;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int k = 4 * h + m;
+          const bool odd = (m & 1) != 0;  // cx and col0 are even
+          float v = (static_cast<float>(raw[k]) - (odd ? b1 : b0)) * (odd ? i1 : i0);
+          v = v > 0.0F ? v : 0.0F;
+          if (lsc) {
+            float gain;
+            if (table) {
+              const F2 e = lsc_tab.e[seg][par][c * kC + k];
+              gain = e.v[0] + e.v[1] * tyr;
+            } else {
+              const int x = cx + k;
+              gain = x >= 0 && x < W ? lsc_gain(f, x, y, cfa_channel(f.cfa, (par << 1) | (x & 1))) : 1.0F;
+            }
+            v = v * gain;
           }
-          v = v * gain;
+          v = v * (odd ? g1 : g0);
+          v = v < 1.0F ? v : 1.0F;
+          o.v[m] = v;
+          if (count && (interior_x || cx + k < W)) saturated += raw[k] >= f.white ? 1U : 0U;
         }
-        v = v * (odd ? g1 : g0);
-        v = v < 1.0F ? v : 1.0F;
-        samp[sr * kSW + col] = v;
-        if (count && x >= x0 && x < x0 + kTile) saturated += raw[k] >= f.white ? 1U : 0U;
+        *reinterpret_cast<F4*>(&samp[sr * kSW + col0]) = o;
       }
     }
     acc[3] += saturated;
-    if (edge_x) {
+    if (!interior_x) {
       // Columns outside the frame take the sample of their reflection.
       __syncthreads();
       for (int i = tid; i < kSH * kSW; i += kThreads) {
         const int sr = i / kSW;
         const int col = i - sr * kSW;
-        const int x = sx0 + col;
+        const int x = x0 - kPad + col;
         if (x >= 0 && x < W) continue;
-        const int src = mirror_near(x, W) - sx0;
-        samp[i] = src >= 0 && src < kSW && src + sx0 < W ? samp[sr * kSW + src] : 0.0F;
+        const int src = mirror_near(x, W) - x0 + kPad;
+        samp[i] = src >= 0 && src < kSW ? samp[sr * kSW + src] : 0.0F;
       }
     }
     __syncthreads();
 
-    // 2. With sharpening: the demosaiced RGB of the tile and a one-pixel ring,
-    // ring pixel (rc, rr) = virtual pixel (x0 - 1 + rc, y0 - 1 + rr), in 4x2 blocks.
+    // 2. With sharpening: the demosaiced RGB of the tile and a one-pixel ring
+    // (ring column rc = x - x0 + 2, row rr = y - y0 + 1), 8 x 2 pixels per thread.
     if constexpr (kSharpen) {
-      constexpr int kBX = kRW / 4;
-      constexpr int kBlocks = kBX * (kRH / 2);
-      for (int b = tid; b < kBlocks; b += kThreads) {
-        const int by = b / kBX;
-        const int bx = b - by * kBX;
-        const int rc = 4 * bx;
-        const int rr = 2 * by;
-        // window: sample rows rr + kH - 1 - kR .. + 1 + kR, columns rc + kH - 1 - kR, 8 wide
+      if (tid < G::kRingBlocks) {
+        const int by = tid / (kRW / 8);
+        const int bx = tid - by * (kRW / 8);
+        const int rc0 = 8 * bx;
+        const int rr0 = 2 * by;
+        // window: sample rows rr0 .. rr0 + 1 + 2R, columns rc0 .. rc0 + 11
         constexpr int kWR = 2 + 2 * kR;
-        constexpr int kWC = 8;
-        float w[kWR][kWC];
-        const int base = (rr + kH - 1 - kR) * kSW + rc + kH - 1 - kR;
+        float w[kWR][12];
 #pragma unroll
         for (int r = 0; r < kWR; ++r) {
-          const F4 a = *reinterpret_cast<const F4*>(&samp[base + r * kSW]);
-          const F4 c = *reinterpret_cast<const F4*>(&samp[base + r * kSW + 4]);
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            w[r][k] = a.v[k];
-            w[r][4 + k] = c.v[k];
+          for (int q = 0; q < 3; ++q) {
+            const F4 a = *reinterpret_cast<const F4*>(&samp[(rr0 + r) * kSW + rc0 + 4 * q]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
           }
         }
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          float rgbs[4][3];
+          float rgbs[8][3];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            // virtual pixel parity: ((1 + i) & 1, (1 + j) & 1)
-            const int ch = cfa_channel(f.cfa, (((1 + j) & 1) << 1) | ((1 + i) & 1));
+          for (int i = 0; i < 8; ++i) {
+            // ring pixel parity: x = x0 - 2 + rc0 + i, y = y0 - 1 + rr0 + j
+            const int ch = cfa_channel(f.cfa, (((1 + j) & 1) << 1) | (i & 1));
             const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
             float first, second;
-            demosaic_pair<kMalvar>(w, kR + j, kR + i, rb, &first, &second);
-            assign_rgb(ch, w[kR + j][kR + i], first, second, rgbs[i]);
+            demosaic_pair<kMalvar>(w, kR + j, 2 + i, rb, &first, &second);
+            assign_rgb(ch, w[kR + j][2 + i], first, second, rgbs[i]);
           }
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            F4 o;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) o.v[i] = rgbs[i][k];
-            *reinterpret_cast<F4*>(&ring[k][(rr + j) * kRW + rc]) = o;
+            for (int q = 0; q < 2; ++q) {
+              F4 o;
+#pragma unroll
+              for (int i = 0; i < 4; ++i) o.v[i] = rgbs[4 * q + i][k];
+              *reinterpret_cast<F4*>(&ring[k][(rr0 + j) * kRW + rc0 + 4 * q]) = o;
+            }
           }
         }
       }
       __syncthreads();
     }
 
-    // 3. A 4x4 block of output pixels per thread.
+    // 3. A 4 x kRows3 block of output pixels per thread.
     {
-      const int bx = tid & 15;
-      const int by = tid >> 4;
-      const int px = 4 * bx;
-      const int py = 4 * by;
+      const int px = 4 * (tid & 15);
+      const int py = kRows3 * (tid >> 4);
       const int xo = x0 + px;
+      float rgb[kRows3][4][3];
+      if constexpr (kSharpen) {
+        // sharpen (isp_math.hpp:219-227) over ring rows py .. py + kRows3 + 1, columns px .. px + 7
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int yo = y0 + py + j;
-        float rgb[4][3];
-        if constexpr (kSharpen) {
-          // ring rows py + j .. + 2, columns px .. px + 5
-          float m[3][3][6];
+        for (int k = 0; k < 3; ++k) {
+          float m[kRows3 + 2][8];
 #pragma unroll
-          for (int k = 0; k < 3; ++k) {
+          for (int r = 0; r < kRows3 + 2; ++r) {
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-              const float* src = &ring[k][(py + j + r) * kRW + px];
-              const F4 a = *reinterpret_cast<const F4*>(src);
-              const F2 b2 = *reinterpret_cast<const F2*>(src + 4);
-              m[k][r][0] = a.v[0];
-              m[k][r][1] = a.v[1];
-              m[k][r][2] = a.v[2];
-              m[k][r][3] = a.v[3];
-              m[k][r][4] = b2.v[0];
-              m[k][r][5] = b2.v[1];
+            for (int q = 0; q < 2; ++q) {
+              const F4 a = *reinterpret_cast<const F4*>(&ring[k][(py + r) * kRW + px + 4 * q]);
+#pragma unroll
+              for (int e = 0; e < 4; ++e) m[r][4 * q + e] = a.v[e];
             }
           }
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
+          for (int j = 0; j < kRows3; ++j) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              // sharpen (isp_math.hpp:219-227)
-              const float c = m[k][1][i + 1];
-              const float blur = (4.0F * c + 2.0F * (m[k][0][i + 1] + m[k][2][i + 1] + m[k][1][i] + m[k][1][i + 2]) +
-                                  (m[k][0][i] + m[k][0][i + 2] + m[k][2][i] + m[k][2][i + 2])) *
+            for (int i = 0; i < 4; ++i) {
+              const float c = m[j + 1][i + 2];
+              const float blur = (4.0F * c + 2.0F * (m[j][i + 2] + m[j + 2][i + 2] + m[j + 1][i + 1] + m[j + 1][i + 3]) +
+                                  (m[j][i + 1] + m[j][i + 3] + m[j + 2][i + 1] + m[j + 2][i + 3])) *
                                  0.0625F;
-              rgb[i][k] = clamp01(c + f.sharpen * (c - blur));
+              rgb[j][i][k] = clamp01(c + f.sharpen * (c - blur));
             }
           }
-        } else {
-          constexpr int kWR = 1 + 2 * kR;
-          constexpr int kWC = 8;
-          float w[kWR][kWC];
-          const int base = (py + j + kH - kR) * kSW + px + kH - kR;
+        }
+      } else {
+        // demosaic from samples rows py + kH - kR .. py + kRows3 - 1 + kH + kR, columns px .. px + 11
+        constexpr int kWR = kRows3 + 2 * kR;
+        float w[kWR][12];
 #pragma unroll
-          for (int r = 0; r < kWR; ++r) {
-            const F4 a = *reinterpret_cast<const F4*>(&samp[base + r * kSW]);
-            const F4 c = *reinterpret_cast<const F4*>(&samp[base + r * kSW + 4]);
+        for (int r = 0; r < kWR; ++r) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              w[r][k] = a.v[k];
-              w[r][4 + k] = c.v[k];
-            }
+          for (int q = 0; q < 3; ++q) {
+            const F4 a = *reinterpret_cast<const F4*>(&samp[(py + kH - kR + r) * kSW + px + 4 * q]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
           }
+        }
+#pragma unroll
+        for (int j = 0; j < kRows3; ++j) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int ch = cfa_channel(f.cfa, ((j & 1) << 1) | (i & 1));
             const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
             float first, second;
-            demosaic_pair<kMalvar>(w, kR, kR + i, rb, &first, &second);
-            assign_rgb(ch, w[kR][kR + i], first, second, rgb[i]);
+            demosaic_pair<kMalvar>(w, kR + j, 4 + i, rb, &first, &second);
+            assign_rgb(ch, w[kR + j][4 + i], first, second, rgb[j][i]);
           }
         }
-        if (yo >= H) continue;
+      }
+      const bool full_x = xo + 4 <= W;
+#pragma unroll
+      for (int j = 0; j < kRows3; ++j) {
+        const int yo = y0 + py + j;
+        if (yo >= H) break;
         // correct_color, to_fixed sums, tone, store, output codes (isp_math.hpp:229-305)
+        float lin[4][3];
         float t[4][3];
         int q[4][3];
-        const bool valid[4] = {xo < W, xo + 1 < W, xo + 2 < W, xo + 3 < W};
+        unsigned int sums[3] = {0U, 0U, 0U};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          float lin[3];
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            lin[k] = clamp01(ccm[3 * k] * rgb[i][0] + ccm[3 * k + 1] * rgb[i][1] + ccm[3 * k + 2] * rgb[i][2]);
+            lin[i][k] = clamp01(f.ccm[3 * k] * rgb[j][i][0] + f.ccm[3 * k + 1] * rgb[j][i][1] +
+                                f.ccm[3 * k + 2] * rgb[j][i][2]);
           }
-          if (want_stats && valid[i]) {
+          if (want_stats && (full_x || xo + i < W)) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) acc[k] += static_cast<unsigned int>(lin[k] * 1048576.0F + 0.5F);
+            for (int k = 0; k < 3; ++k) sums[k] += static_cast<unsigned int>(lin[i][k] * 1048576.0F + 0.5F);
           }
+        }
+        if (lut_lds) {
 #pragma unroll
-          for (int k = 0; k < 3; ++k) t[i][k] = tone(lut, lut_size, lin[k]);
+          for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) t[i][k] = tone(lut_s, n1, lut_top, lin[i][k]);
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) t[i][k] = tone(f.lut, n1, lut_top, lin[i][k]);
+          }
+        }
+        if (want_stats) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) acc[k] += sums[k];
         }
         std::uint8_t* orow = f.out + static_cast<long long>(yo) * f.out_pitch;
-        const bool whole = out_aligned != 0 && xo + 4 <= W;
+        const bool whole = out_aligned != 0 && full_x;
         switch (f.out_format) {
           case ISP_OUT_RGB8: {
             unsigned int byte[12];
@@ -878,7 +969,7 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
             } else {
 #pragma unroll
               for (int i = 0; i < 4; ++i) {
-                if (!valid[i]) continue;
+                if (xo + i >= W) continue;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) orow[3 * (xo + i) + k] = static_cast<std::uint8_t>(byte[3 * i + k]);
               }
@@ -905,7 +996,7 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
             } else {
 #pragma unroll
               for (int i = 0; i < 4; ++i) {
-                if (!valid[i]) continue;
+                if (xo + i >= W) continue;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) orow[4 * (xo + i) + k] = static_cast<std::uint8_t>(word[i] >> (8 * k));
               }
@@ -934,12 +1025,11 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
             } else {
 #pragma unroll
               for (int i = 0; i < 4; ++i) {
-                if (!valid[i]) continue;
+                if (xo + i >= W) continue;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                  const std::uint16_t v = static_cast<std::uint16_t>(half[3 * i + k]);
-                  orow[6 * (xo + i) + 2 * k] = static_cast<std::uint8_t>(v & 0xFF);
-                  orow[6 * (xo + i) + 2 * k + 1] = static_cast<std::uint8_t>(v >> 8);
+                  orow[6 * (xo + i) + 2 * k] = static_cast<std::uint8_t>(half[3 * i + k] & 0xFFU);
+                  orow[6 * (xo + i) + 2 * k + 1] = static_cast<std::uint8_t>(half[3 * i + k] >> 8);
                 }
               }
             }
@@ -964,7 +1054,7 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
             } else {
 #pragma unroll
               for (int i = 0; i < 4; ++i) {
-                if (!valid[i]) continue;
+                if (xo + i >= W) continue;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) reinterpret_cast<float*>(orow)[3 * (xo + i) + k] = t[i][k];
               }
@@ -975,19 +1065,12 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
         if (want_hist) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            if (!valid[i]) continue;
+            if (!full_x && xo + i >= W) continue;
             const int luma = (2126 * q[i][0] + 7152 * q[i][1] + 722 * q[i][2] + 5000) / 10000;
-            if (!wide) {
-              atomicAdd(&hist[luma], 1U);
-              atomicAdd(&hist[ISP_HISTOGRAM_BINS + q[i][0]], 1U);
-              atomicAdd(&hist[2 * ISP_HISTOGRAM_BINS + q[i][1]], 1U);
-              atomicAdd(&hist[3 * ISP_HISTOGRAM_BINS + q[i][2]], 1U);
-            } else {
-              wide_token ^= atomicAdd(&s->main.hist[0][luma], 1U);
-              wide_token ^= atomicAdd(&s->main.hist[1][q[i][0]], 1U);
-              wide_token ^= atomicAdd(&s->main.hist[2][q[i][1]], 1U);
-              wide_token ^= atomicAdd(&s->main.hist[3][q[i][2]], 1U);
-            }
+            atomicAdd(&hist[luma * copies + copy], 1U);
+            atomicAdd(&hist[(bins + q[i][0]) * copies + copy], 1U);
+            atomicAdd(&hist[(2 * bins + q[i][1]) * copies + copy], 1U);
+            atomicAdd(&hist[(3 * bins + q[i][2]) * copies + copy], 1U);
           }
         }
       }
@@ -1004,15 +1087,17 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
     }
   }
   __syncthreads();
-  unsigned int token = wide_token;
+  unsigned int token = 0;
   if (want_stats && tid < 4) {
     unsigned long long total = 0;
     for (int w = 0; w < kThreads / 64; ++w) total += part[w][tid];
     if (total != 0) token ^= fold(atomicAdd(&s->main.sum[tid], total));
   }
-  if (want_hist && !wide) {
-    for (int i = tid; i < 4 * ISP_HISTOGRAM_BINS; i += kThreads) {
-      if (hist[i] != 0U) token ^= atomicAdd(&s->main.hist[i / ISP_HISTOGRAM_BINS][i % ISP_HISTOGRAM_BINS], hist[i]);
+  if (want_hist) {
+    for (int i = tid; i < 4 * bins; i += kThreads) {
+      unsigned int total = 0;
+      for (int c = 0; c < copies; ++c) total += hist[i * copies + c];
+      if (total != 0U) token ^= atomicAdd(&s->main.hist[i / bins][i % bins], total);
     }
   }
   if (!last_block(&s->main.done, token)) return;
@@ -1063,11 +1148,13 @@ bool out_aligned(const FrameDesc& f) {
   return ((p | static_cast<unsigned long long>(f.out_pitch)) & 15ULL) == 0;
 }
 
+constexpr int kMainTileH = 32;
+
 template <int kFmt, bool kMalvar, bool kSharpen>
 void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream) {
-  auto kernel = main_kernel<kFmt, kMalvar, kSharpen>;
-  const int tiles_x = (f.width + kTile - 1) / kTile;
-  const int tiles_y = (f.height + kTile - 1) / kTile;
+  auto kernel = main_kernel<kFmt, kMalvar, kSharpen, kMainTileH>;
+  const int tiles_x = (f.width + 63) / 64;
+  const int tiles_y = (f.height + kMainTileH - 1) / kMainTileH;
   const int tiles = tiles_x * tiles_y;
   const int grid = std::max(1, std::min(tiles, cfg.cu_count * blocks_per_cu(kernel)));
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, f, s, tiles_x, tiles, rows_aligned(f) ? 1 : 0,
@@ -1088,10 +1175,10 @@ void launch_main_f(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharp
 template <int kFmt>
 void launch_awb_f(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream) {
   auto kernel = awb_kernel<kFmt>;
-  const int tiles_x = (f.width + kTile - 1) / kTile;
-  const int tiles_y = (f.height + kTile - 1) / kTile;
+  const int tiles_x = (f.width + kAwbTile - 1) / kAwbTile;
+  const int tiles_y = (f.height + kAwbTile - 1) / kAwbTile;
   const int tiles = tiles_x * tiles_y;
-  const int grid = std::max(1, std::min(tiles, cfg.cu_count * blocks_per_cu(kernel)));
+  const int grid = std::max(1, std::min(tiles, cfg.cu_count * std::min(4, blocks_per_cu(kernel))));
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, f, s, tiles_x, tiles, rows_aligned(f) ? 1 : 0);
   check(hipGetLastError(), "awb_kernel launch");
 }
