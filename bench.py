@@ -33,7 +33,45 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-frames", type=int, default=6, help="frames of the single-thread CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (10 times): the reference "
+                         "point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size; under rocprofv3 --pmc its counters
+    per dispatch give the KiB the counter reports for mib MiB read and mib MiB written."""
+    import torch
+    torch.cuda.set_device(0)
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty(n, dtype=torch.uint8, device="cuda")
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "launches": 10}))
+
+
+def traffic_from_counters():
+    """HBM bytes per main-kernel launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    readings, each scaled by what the same counter reads for a 256 MiB device copy. Also the
+    fetch scaled by the white-balance kernel, which reads the packed frame exactly once with the
+    same vector loads."""
+    path = os.path.join(ROOT, "profiles", "main_kernel_4k_raw12_malvar_counters.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        c = json.load(fh)
+    fetch_factor = c["copy_bytes"] / 1024.0 / c["copy_fetch_kib"]
+    write_factor = c["copy_bytes"] / 1024.0 / c["copy_write_kib"]
+    copy_scaled = (c["fetch_kib_per_dispatch"] * fetch_factor + c["write_kib_per_dispatch"] * write_factor) * 1024.0
+    awb_factor = c["awb_algorithmic_bytes"] / 1024.0 / c["awb_fetch_kib_per_dispatch"]
+    awb_scaled = (c["fetch_kib_per_dispatch"] * awb_factor + c["write_kib_per_dispatch"] * write_factor) * 1024.0
+    detail = {"fetch_factor_copy": round(fetch_factor, 4), "write_factor_copy": round(write_factor, 4),
+              "fetch_factor_single_read_kernel": round(awb_factor, 4),
+              "traffic_single_read_scaled": round(awb_scaled), "source": os.path.relpath(path, ROOT)}
+    return round(copy_scaled), detail
 
 
 def percentile(values, q):
@@ -83,6 +121,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
+        return
     import numpy as np
     import torch
 
@@ -152,6 +193,7 @@ def main():
     bit_exact = bool(gs.mismatches(g1, h1) == 0 and np.array_equal(g1.histogram, h1.histogram)
                      and gs.same_stats(g1, h1))
 
+    traffic, traffic_detail = traffic_from_counters()
     result = {
         "metric": "Gpix/s + HBM GB/s (bit-exact); p99 frame latency, batched vs per-frame",
         "value": round(gpix, 3),
@@ -177,8 +219,9 @@ def main():
         "kernel_ms": {"awb_avg": round(awb_ms / n, 5), "main_avg": round(main_ms / n, 5)},
         "bit_exact": bit_exact,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "main_kernel<malvar,sharpen>", "algorithmic_bytes_per_launch": algo_bytes},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "main_kernel<malvar,sharpen>", "algorithmic_bytes_per_launch": algo_bytes,
+                     "traffic_detail": traffic_detail},
     }
     if not args.no_cpu:
         base = cpu_baseline(frame, params, out_format, args.cpu_frames)
