@@ -41,6 +41,14 @@ std::size_t raw_row_bytes(int format, int width);
 void set_error(const std::string& message);
 void clear_error();
 
+// A camera's exposure carried on the device from one frame to the next (served
+// form): the next frame runs at the previous frame's suggestion.
+struct AeState {
+  long long exposure_ns;
+  float gain;
+  int valid;  // 0: use the frame's own exposure
+};
+
 // Everything one frame's kernels read, passed by value (FrameParams,
 // isp_math.hpp:35-60, plus the outputs and the statistics switches).
 struct FrameDesc {
@@ -78,6 +86,9 @@ struct FrameDesc {
   int wb_mode;
   float percentile;
   float smoothing;
+  // Optional device exposure state: read instead of exposure_ns / exposure_gain
+  // once valid, and replaced by this frame's suggestion.
+  AeState* ae_state;
 };
 
 // Per-pipeline device scratch, zero between frames (each pass's last block clears
@@ -105,6 +116,55 @@ struct DeviceScratch {
   MainScratch main;
 };
 
+// ---- the batched form: many frames in one persistent launch ----------------
+
+inline constexpr int kMaxBatch = 64;      // frames per launch
+inline constexpr int kBatchTileW = 64;    // main-pass tile width
+inline constexpr int kBatchAwbTile = 64;  // white-balance tile (samples, square)
+
+// One frame of a batch. The variant selects the tile body: raw format (bits 0-1),
+// Malvar (bit 2), sharpening (bit 3), 16-row tiles instead of 32 (bit 4).
+struct BatchFrame {
+  FrameDesc f;
+  DeviceScratch* s;  // the frame's stream: white balance state and scratch
+  int variant;
+  int tiles_x;  // main pass: tiles across, tiles, tiles per work item, work items
+  int tiles;
+  int tiles_per_item;
+  int items;
+  int awb_tiles_x;  // white balance pass; 0 items when the balance is manual
+  int awb_tiles;
+  int awb_tiles_per_item;
+  int awb_items;
+  int fast_rows;
+  int out_aligned;
+};
+
+// The work queue's layout: all white-balance items first (frames in awb_order),
+// then all main-pass items (frames in main_order). start[k] is the first item
+// of the k-th frame in that order, relative to the start of its pass.
+struct BatchTable {
+  int count;
+  int awb_items;
+  int total_items;
+  int pad;
+  int awb_start[kMaxBatch];
+  int awb_order[kMaxBatch];
+  int main_start[kMaxBatch];
+  int main_order[kMaxBatch];
+  BatchFrame frames[kMaxBatch];
+};
+
+// Device control block of a batch, zero at creation; every launch leaves it
+// ready for the next (the last block to exit resets the queue).
+struct BatchControl {
+  unsigned int next;              // work queue head
+  unsigned int exited;            // blocks that left the queue
+  unsigned int tick;              // launches completed
+  unsigned int error;             // set when a bounded wait gave up
+  unsigned int ready[kMaxBatch];  // tick + 1 once frame i's white balance is final
+};
+
 // Kernel launchers (isp_kernels.hip). `stream` is a hipStream_t.
 struct LaunchConfig {
   int cu_count;
@@ -114,5 +174,10 @@ struct LaunchConfig {
 void launch_awb(const FrameDesc& f, DeviceScratch* scratch, const LaunchConfig& cfg, void* stream);
 void launch_main(const FrameDesc& f, DeviceScratch* scratch, bool malvar, bool sharpen,
                  const LaunchConfig& cfg, void* stream);
+// The batch kernel's resident blocks per CU, and its launch (grid blocks).
+int batch_blocks_per_cu();
+void launch_batch(const BatchTable* table, BatchControl* control, int grid, void* stream);
+// Bounded wait of the batch kernel, in sleep rounds (tests shorten it).
+void set_batch_spin_limit(unsigned int rounds);
 
 }  // namespace isp_amd

@@ -1,6 +1,6 @@
 // isp_kernels.hip - the ISP's gfx950 kernels.
 //
-// Two launches per frame, one when the white balance is manual:
+// Per-frame form, two launches per frame (one when the white balance is manual):
 //
 //  awb_kernel   white balance statistics (reference: white_balance_quad,
 //               isp_math.hpp:311-332; awb_kernel, cuda_pipeline.cu:119-158).
@@ -32,6 +32,18 @@
 //               (finish_stats, isp_math.hpp:387-425, with the exact log2/exp2
 //               of isp_ae.hpp).
 //
+// Batched form, one launch for many frames (batch_kernel): a persistent grid
+// takes work items (a few tiles of one frame) from a device queue that lists
+// every frame's white-balance items first and then every frame's main-pass
+// items. The same tile bodies run per item, chosen per frame among all raw
+// formats, both demosaics, with and without sharpening and two tile heights. A
+// block keeps one frame's partial sums and histograms in LDS while it takes
+// consecutive items of that frame, and adds them to the frame's scratch when it
+// moves on; the block whose contribution completes a frame's item count runs
+// that frame's epilogue. A main-pass item of an auto-white-balanced frame waits
+// (bounded) for that frame's gains: every white-balance item was taken before
+// any main-pass item, by a block that never waits, so the wait always ends.
+//
 // Lens shading: the reference's per-sample bilinear interpolation
 // (lens_shading_gain, isp_math.hpp:121-140) is top + (bottom - top) * ty with
 // top and bottom depending only on the column and the grid row. Each tile
@@ -45,6 +57,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -136,25 +150,36 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // partial sums with device-scope atomics whose results each thread consumes
 // (folded into `token`), so every atomic has been performed at the point of
 // coherence before the thread reaches the barrier; only then does one thread
-// take a ticket. The last block reads the totals back with atomic exchanges,
-// which also clear them for the next frame. (An agent-scope fence here would
-// write back and invalidate the whole L2 of the block's XCD, once per block.)
-__device__ __forceinline__ bool last_block(unsigned int* done, unsigned int token) {
-  __shared__ int is_last;
+// add the block's `count` contributions to the counter. The block whose
+// addition reaches `total` is the last one: it reads the totals back with
+// atomic exchanges, which also clear them for the next frame. (An agent-scope
+// fence here would write back and invalidate the whole L2 of the block's XCD.)
+// `flag` is a block-shared int.
+__device__ __forceinline__ bool complete(unsigned int* done, unsigned int count, unsigned int total,
+                                         unsigned int token, int* flag) {
   // Consuming the token makes the atomics return their values and the wave wait
   // for them here (the asm reads a register and touches no memory).
   __asm__ volatile("" : : "v"(token));
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned int ticket = atomicAdd(done, 1U);
-    is_last = ticket == gridDim.x - 1 ? 1 : 0;
+    const unsigned int before = atomicAdd(done, count);
+    *flag = before + count == total ? 1 : 0;
   }
   __syncthreads();
-  return is_last != 0;
+  return *flag != 0;
 }
 
 __device__ __forceinline__ unsigned int fold(unsigned long long v) {
   return static_cast<unsigned int>(v) ^ static_cast<unsigned int>(v >> 32);
+}
+
+// A float read and written at the point of coherence (state shared by blocks of
+// one launch).
+__device__ __forceinline__ float coherent_read(const float* p) {
+  return __uint_as_float(atomicOr(reinterpret_cast<unsigned int*>(const_cast<float*>(p)), 0U));
+}
+__device__ __forceinline__ unsigned int coherent_write(float* p, float v) {
+  return atomicExch(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
 }
 
 // ---- lens shading ---------------------------------------------------------
@@ -347,188 +372,238 @@ __device__ float percentile_wave(const unsigned int* hist, float percentile) {
   return static_cast<float>((static_cast<double>(bin) + w) / kAwbBins);
 }
 
-constexpr int kAwbTile = 64;
+constexpr int kAwbTile = kBatchAwbTile;
 
+// LDS of the white balance pass: per frame, and per tile.
+struct AwbFrameLds {
+  unsigned int hist[3 * kAwbBins];
+  float grid[kMaxGrid];
+  unsigned long long part[kThreads / 64][4];
+  unsigned long long sums[4];
+  float level[3];
+  int flag;
+};
+struct AwbTileLds {
+  LscTable<kAwbTile> lsc;
+};
+
+// A block's constants for one frame of the white balance pass.
+struct AwbCtx {
+  QuadConst qc;
+  int colour[4];
+  const float* grid;
+  bool patch;
+  bool lsc;
+};
+
+// Starts a block's work on a frame. The tile's first barrier orders these LDS
+// writes before any reader.
+__device__ __forceinline__ void awb_begin(const FrameDesc& f, AwbFrameLds& fl, AwbCtx& c) {
+  c.patch = f.wb_mode == ISP_WB_WHITE_PATCH;
+  if (c.patch) {
+    for (int i = threadIdx.x; i < 3 * kAwbBins; i += kThreads) fl.hist[i] = 0U;
+  }
+  c.grid = stage_grid(f, fl.grid);
+  const float ones[3] = {1.0F, 1.0F, 1.0F};
+  c.qc = quad_consts(f, ones);
+#pragma unroll
+  for (int p = 0; p < 4; ++p) c.colour[p] = color_of(cfa_channel(f.cfa, p));
+  c.lsc = f.lsc != nullptr;
+}
+
+// One 64 x 64 sample tile: white_balance_quad (isp_math.hpp:311-332) over its
+// complete quads, sums into local[0..2], quads into local[3].
 template <int kFmt>
-__global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles,
-                                                       int fast_rows) {
+__device__ __forceinline__ void awb_tile(const FrameDesc& f, const AwbCtx& c, AwbFrameLds& fl, AwbTileLds& tl,
+                                         int tile, int tiles_x, int fast_rows, unsigned long long local[4]) {
   constexpr int kC = Chunk<kFmt>::kSamples;
-  constexpr int kCols = kAwbTile;
   constexpr int kTasksX = kAwbTile / kC;
   constexpr int kTasks = kTasksX * (kAwbTile / 2);
   static_assert(kTasks <= kThreads, "one task per thread");
-  __shared__ unsigned int hist[3 * kAwbBins];
-  __shared__ unsigned long long part[kThreads / 64][4];
-  __shared__ __attribute__((aligned(16))) LscTable<kCols> lsc_tab;
-  __shared__ float grid_s[kMaxGrid];
-  const bool patch = f.wb_mode == ISP_WB_WHITE_PATCH;
   const int tid = threadIdx.x;
-  if (patch) {
-    for (int i = tid; i < 3 * kAwbBins; i += kThreads) hist[i] = 0U;
-  }
-  const float* grid = stage_grid(f, grid_s);
-  QuadConst qc;
-  {
-    const float ones[3] = {1.0F, 1.0F, 1.0F};
-    qc = quad_consts(f, ones);
-  }
-  int colour[4];
-#pragma unroll
-  for (int p = 0; p < 4; ++p) colour[p] = color_of(cfa_channel(f.cfa, p));
   const int W = f.width;
   const int H = f.height;
   const int qw = W / 2;
   const int qh = H / 2;
-  const bool lsc = f.lsc != nullptr;
-  unsigned long long local[4] = {0, 0, 0, 0};
-
-  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const int tyi = tile / tiles_x;
-    const int x0 = (tile - tyi * tiles_x) * kAwbTile;
-    const int y0 = tyi * kAwbTile;
-    const int qr = tid / kTasksX;
-    const int cx = x0 + (tid - qr * kTasksX) * kC;
-    const int qy = (y0 >> 1) + qr;
-    const bool active = tid < kTasks && qy < qh && cx < 2 * qw;
-    const bool whole = active && fast_rows != 0 && cx + kC <= W;
-    const std::uint8_t* row0 = f.raw + static_cast<long long>(2 * qy) * f.raw_pitch;
-    const std::uint8_t* row1 = row0 + f.raw_pitch;
-    Packed<kFmt> pk[2];
-    if (whole) {
-      load_packed<kFmt>(row0, cx, pk[0]);
-      load_packed<kFmt>(row1, cx, pk[1]);
-    }
-    int lo = 0;
-    bool table = false;
-    __syncthreads();  // the previous tile's table readers are done
-    if (lsc) {
-      lo = lsc_row(f, y0);
-      table = lsc_row(f, min(y0 + kAwbTile - 1, H - 1)) - lo <= 1;
-      if (table) build_lsc_table<kCols>(f, grid, x0, lo, lsc_tab);
-    }
-    __syncthreads();
-    if (!active) continue;
-    int raw[2][kC];
-    if (whole) {
-      unpack<kFmt>(pk[0], raw[0]);
-      unpack<kFmt>(pk[1], raw[1]);
-    } else {
-#pragma unroll
-      for (int k = 0; k < kC; ++k) {
-        raw[0][k] = cx + k < 2 * qw ? load_raw(row0, kFmt, cx + k) : 0;
-        raw[1][k] = cx + k < 2 * qw ? load_raw(row1, kFmt, cx + k) : 0;
-      }
-    }
-    float tyr[2] = {0.0F, 0.0F};
-    int seg[2] = {0, 0};
-    if (table) {
-#pragma unroll
-      for (int r = 0; r < 2; ++r) seg[r] = lsc_row(f, 2 * qy + r, &tyr[r]) - lo;
-    }
-    unsigned int sum[3] = {0U, 0U, 0U};
-    unsigned int quads = 0U;
-#pragma unroll
-    for (int q = 0; q < kC / 2; ++q) {
-      if ((cx >> 1) + q >= qw) continue;
-      bool ok = true;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) ok = ok && raw[k >> 1][2 * q + (k & 1)] < f.white;
-      if (!ok) continue;
-      quads += 1U;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int r = k >> 1;
-        const int c = 2 * q + (k & 1);
-        // white_balance_quad (isp_math.hpp:311-332): linearize, cap at 4, fixed point, bin
-        float v = (static_cast<float>(raw[r][c]) - qc.black[k]) * qc.inv[k];
-        v = v > 0.0F ? v : 0.0F;
-        if (lsc) {
-          float gain;
-          if (table) {
-            const F2 e = lsc_tab.e[seg[r]][r][cx - x0 + c];
-            gain = e.v[0] + e.v[1] * tyr[r];
-          } else {
-            gain = lsc_gain(f, cx + c, 2 * qy + r, cfa_channel(f.cfa, k));
-          }
-          v = v * gain;
-        }
-        v = v < 4.0F ? v : 4.0F;
-        const int col = colour[k];
-        const unsigned int fixed = static_cast<unsigned int>(v * 1048576.0F + 0.5F);
-        sum[0] += col == 0 ? fixed : 0U;
-        sum[1] += col == 1 ? fixed : 0U;
-        sum[2] += col == 2 ? fixed : 0U;
-        if (patch) {
-          const int b = static_cast<int>(v * static_cast<float>(kAwbBins));
-          atomicAdd(&hist[col * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) local[k] += sum[k];
-    local[3] += quads;
+  const int tyi = tile / tiles_x;
+  const int x0 = (tile - tyi * tiles_x) * kAwbTile;
+  const int y0 = tyi * kAwbTile;
+  const int qr = tid / kTasksX;
+  const int cx = x0 + (tid - qr * kTasksX) * kC;
+  const int qy = (y0 >> 1) + qr;
+  const bool active = tid < kTasks && qy < qh && cx < 2 * qw;
+  const bool whole = active && fast_rows != 0 && cx + kC <= W;
+  const std::uint8_t* row0 = f.raw + static_cast<long long>(2 * qy) * f.raw_pitch;
+  const std::uint8_t* row1 = row0 + f.raw_pitch;
+  Packed<kFmt> pk[2];
+  if (whole) {
+    load_packed<kFmt>(row0, cx, pk[0]);
+    load_packed<kFmt>(row1, cx, pk[1]);
   }
+  int lo = 0;
+  bool table = false;
+  __syncthreads();  // the previous tile's table readers are done
+  if (c.lsc) {
+    lo = lsc_row(f, y0);
+    table = lsc_row(f, min(y0 + kAwbTile - 1, H - 1)) - lo <= 1;
+    if (table) build_lsc_table<kAwbTile>(f, c.grid, x0, lo, tl.lsc);
+  }
+  __syncthreads();
+  if (!active) return;
+  int raw[2][kC];
+  if (whole) {
+    unpack<kFmt>(pk[0], raw[0]);
+    unpack<kFmt>(pk[1], raw[1]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kC; ++k) {
+      raw[0][k] = cx + k < 2 * qw ? load_raw(row0, kFmt, cx + k) : 0;
+      raw[1][k] = cx + k < 2 * qw ? load_raw(row1, kFmt, cx + k) : 0;
+    }
+  }
+  float tyr[2] = {0.0F, 0.0F};
+  int seg[2] = {0, 0};
+  if (table) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) seg[r] = lsc_row(f, 2 * qy + r, &tyr[r]) - lo;
+  }
+  unsigned int sum[3] = {0U, 0U, 0U};
+  unsigned int quads = 0U;
+#pragma unroll
+  for (int q = 0; q < kC / 2; ++q) {
+    if ((cx >> 1) + q >= qw) continue;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ok = ok && raw[k >> 1][2 * q + (k & 1)] < f.white;
+    if (!ok) continue;
+    quads += 1U;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int r = k >> 1;
+      const int col = 2 * q + (k & 1);
+      // white_balance_quad (isp_math.hpp:311-332): linearize, cap at 4, fixed point, bin
+      float v = (static_cast<float>(raw[r][col]) - c.qc.black[k]) * c.qc.inv[k];
+      v = v > 0.0F ? v : 0.0F;
+      if (c.lsc) {
+        float gain;
+        if (table) {
+          const F2 e = tl.lsc.e[seg[r]][r][cx - x0 + col];
+          gain = e.v[0] + e.v[1] * tyr[r];
+        } else {
+          gain = lsc_gain(f, cx + col, 2 * qy + r, cfa_channel(f.cfa, k));
+        }
+        v = v * gain;
+      }
+      v = v < 4.0F ? v : 4.0F;
+      const int colour = c.colour[k];
+      const unsigned int fixed = static_cast<unsigned int>(v * 1048576.0F + 0.5F);
+      sum[0] += colour == 0 ? fixed : 0U;
+      sum[1] += colour == 1 ? fixed : 0U;
+      sum[2] += colour == 2 ? fixed : 0U;
+      if (c.patch) {
+        const int b = static_cast<int>(v * static_cast<float>(kAwbBins));
+        atomicAdd(&fl.hist[colour * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) local[k] += sum[k];
+  local[3] += quads;
+}
+
+// Adds the block's partial sums and histograms to the frame's scratch; returns
+// the token of the atomics (see complete()).
+__device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl, const AwbCtx& c,
+                                                  const unsigned long long local[4]) {
+  const int tid = threadIdx.x;
   const int wave = tid >> 6;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const unsigned long long v = wave_sum(local[k]);
-    if ((tid & 63) == 0) part[wave][k] = v;
+    if ((tid & 63) == 0) fl.part[wave][k] = v;
   }
   __syncthreads();
   unsigned int token = 0;
   if (tid < 4) {
     unsigned long long total = 0;
-    for (int w = 0; w < kThreads / 64; ++w) total += part[w][tid];
-    if (total != 0) token ^= fold(atomicAdd(tid < 3 ? &s->awb.sum[tid] : &s->awb.quads, total));
+    for (int w = 0; w < kThreads / 64; ++w) total += fl.part[w][tid];
+    if (total != 0) token ^= fold(atomicAdd(tid < 3 ? &a->sum[tid] : &a->quads, total));
   }
-  if (patch) {
+  if (c.patch) {
     for (int i = tid; i < 3 * kAwbBins; i += kThreads) {
-      if (hist[i] != 0U) token ^= atomicAdd(&s->awb.hist[i / kAwbBins][i % kAwbBins], hist[i]);
+      if (fl.hist[i] != 0U) token ^= atomicAdd(&a->hist[i / kAwbBins][i % kAwbBins], fl.hist[i]);
     }
   }
-  if (!last_block(&s->awb.done, token)) return;
+  return token;
+}
 
-  // The last block: finish_white_balance (isp_math.hpp:357-384); the reads clear.
-  __shared__ float level_s[3];
-  __shared__ unsigned long long sums_s[4];
-  if (tid < 4) sums_s[tid] = atomicExch(tid < 3 ? &s->awb.sum[tid] : &s->awb.quads, 0ULL);
-  if (patch) {
-    for (int i = tid; i < 3 * kAwbBins; i += kThreads) hist[i] = atomicExch(&s->awb.hist[i / kAwbBins][i % kAwbBins], 0U);
+// The frame's last contributor: finish_white_balance (isp_math.hpp:357-384)
+// into the stream's white balance state; the reads clear the scratch. With
+// `ready`, the final gains are published there as `ready_value` once written.
+__device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s, AwbFrameLds& fl, const AwbCtx& c,
+                                           unsigned int* ready, unsigned int ready_value) {
+  const int tid = threadIdx.x;
+  if (tid < 4) fl.sums[tid] = atomicExch(tid < 3 ? &s->awb.sum[tid] : &s->awb.quads, 0ULL);
+  if (c.patch) {
+    for (int i = tid; i < 3 * kAwbBins; i += kThreads) fl.hist[i] = atomicExch(&s->awb.hist[i / kAwbBins][i % kAwbBins], 0U);
   }
   __syncthreads();
-  const unsigned long long quads_s = sums_s[3];
+  const unsigned long long quads_s = fl.sums[3];
   if (quads_s != 0) {
-    if (patch) {
+    if (c.patch) {
       if (tid < 64) {
         for (int k = 0; k < 3; ++k) {
-          const float level = percentile_wave(&hist[k * kAwbBins], f.percentile);
-          if (tid == 0) level_s[k] = level;
+          const float level = percentile_wave(&fl.hist[k * kAwbBins], f.percentile);
+          if (tid == 0) fl.level[k] = level;
         }
       }
     } else if (tid == 0) {
       const double quads = static_cast<double>(quads_s);
-      level_s[0] = static_cast<float>(static_cast<double>(sums_s[0]) / quads);
-      level_s[1] = static_cast<float>(static_cast<double>(sums_s[1]) / (2.0 * quads));
-      level_s[2] = static_cast<float>(static_cast<double>(sums_s[2]) / quads);
+      fl.level[0] = static_cast<float>(static_cast<double>(fl.sums[0]) / quads);
+      fl.level[1] = static_cast<float>(static_cast<double>(fl.sums[1]) / (2.0 * quads));
+      fl.level[2] = static_cast<float>(static_cast<double>(fl.sums[2]) / quads);
     }
   }
   __syncthreads();
-  if (tid == 0 && quads_s != 0) {
-    const float level[3] = {level_s[0], level_s[1], level_s[2]};
-    if (level[0] > 0.0F && level[1] > 0.0F && level[2] > 0.0F) {
-      float target[3] = {level[1] / level[0], 1.0F, level[1] / level[2]};
-      for (int k = 0; k < 3; ++k) {
-        target[k] = target[k] < 0.0625F ? 0.0625F : target[k] > 16.0F ? 16.0F : target[k];
+  if (tid == 0) {
+    unsigned int token = 0;
+    if (quads_s != 0) {
+      const float level[3] = {fl.level[0], fl.level[1], fl.level[2]};
+      if (level[0] > 0.0F && level[1] > 0.0F && level[2] > 0.0F) {
+        float target[3] = {level[1] / level[0], 1.0F, level[1] / level[2]};
+        for (int k = 0; k < 3; ++k) {
+          target[k] = target[k] < 0.0625F ? 0.0625F : target[k] > 16.0F ? 16.0F : target[k];
+        }
+        // The state was last written by an earlier launch.
+        WbState& st = s->wb;
+        const bool smooth = st.valid != 0 && f.smoothing < 1.0F;
+        for (int k = 0; k < 3; ++k) {
+          const float g = smooth ? st.gains[k] + f.smoothing * (target[k] - st.gains[k]) : target[k];
+          token ^= coherent_write(&st.gains[k], g);
+        }
+        token ^= atomicExch(reinterpret_cast<unsigned int*>(&st.valid), 1U);
       }
-      WbState& st = s->wb;
-      for (int k = 0; k < 3; ++k) {
-        st.gains[k] = st.valid != 0 && f.smoothing < 1.0F ? st.gains[k] + f.smoothing * (target[k] - st.gains[k])
-                                                          : target[k];
-      }
-      st.valid = 1;
     }
+    if (ready != nullptr) {
+      __asm__ volatile("" : : "v"(token));  // the gains are written before they are announced
+      atomicExch(ready, ready_value);
+    }
+    atomicExch(&s->awb.done, 0U);
   }
-  if (tid == 0) atomicExch(&s->awb.done, 0U);
+}
+
+template <int kFmt>
+__global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles,
+                                                       int fast_rows) {
+  __shared__ AwbFrameLds fl;
+  __shared__ __attribute__((aligned(16))) AwbTileLds tl;
+  AwbCtx c;
+  awb_begin(f, fl, c);
+  unsigned long long local[4] = {0, 0, 0, 0};
+  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) awb_tile<kFmt>(f, c, fl, tl, tile, tiles_x, fast_rows, local);
+  const unsigned int token = awb_flush(&s->awb, fl, c, local);
+  if (!complete(&s->awb.done, 1U, gridDim.x, token, &fl.flag)) return;
+  awb_finish(f, s, fl, c, nullptr, 0U);
 }
 
 // ---- the main pass ----------------------------------------------------------
@@ -637,11 +712,75 @@ struct MainGeom {
   static constexpr int kRows3 = kTileH / 16;
   static constexpr int kLscCols = kNC * kC;
   static_assert(kRingBlocks <= kThreads || !kSharpen, "one ring block per thread");
+  static_assert(kTileH == 16 || kTileH == 32, "tile heights");
 };
 
+// LDS of the main pass: per frame (tone table, histograms, lens shading grid,
+// reductions), and per tile (samples, demosaiced ring, lens shading table).
+struct MainFrameLds {
+  float lut[kLdsLut];
+  unsigned int hist[4 * kMaxHistBins];
+  float grid[kMaxGrid];
+  unsigned long long part[kThreads / 64][4];
+  unsigned long long sums[4];
+  float gains[3];
+  int flag;
+};
 template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
-__global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles,
-                                                        int fast_rows, int out_aligned) {
+struct MainTileLds {
+  using G = MainGeom<kFmt, kMalvar, kSharpen, kTileH>;
+  alignas(16) float samp[G::kSH * G::kSW];
+  alignas(16) float ring[3][G::kRH * G::kRW];
+  alignas(16) LscTable<G::kLscCols> lsc;
+};
+
+// A block's constants for one frame of the main pass.
+struct MainCtx {
+  QuadConst qc;
+  float gains[3];
+  const float* grid;
+  float lut_top;
+  int n1;
+  int bins;
+  int copies;  // interleaved histogram copies: 4 for 256 bins
+  int copy;
+  bool lsc;
+  bool lut_lds;
+  bool want_hist;
+  bool want_stats;
+};
+
+// Starts a block's work on a frame. The tile's first barrier orders these LDS
+// writes before any reader.
+__device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains[3], MainFrameLds& fl, MainCtx& c) {
+  const int tid = threadIdx.x;
+  c.want_hist = f.histogram != nullptr;
+  c.want_stats = f.stats != nullptr || f.ae_state != nullptr;
+  c.bins = f.bins;
+  c.copies = kMaxHistBins / c.bins;
+  c.copy = (tid & 63) & (c.copies - 1);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c.gains[k] = gains[k];
+  c.qc = quad_consts(f, gains);
+  if (c.want_hist) {
+    for (int i = tid; i < 4 * kMaxHistBins; i += kThreads) fl.hist[i] = 0U;
+  }
+  c.lut_lds = f.lut_size <= kLdsLut;
+  if (c.lut_lds) {
+    for (int i = tid; i < f.lut_size; i += kThreads) fl.lut[i] = f.lut[i];
+  }
+  c.n1 = f.lut_size - 1;
+  c.lut_top = f.lut[c.n1];
+  c.lsc = f.lsc != nullptr;
+  c.grid = stage_grid(f, fl.grid);
+}
+
+// One kTileH x 64 output tile; statistics sums into acc[0..2], saturated
+// samples into acc[3].
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
+__device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_, MainFrameLds& fl,
+                                          MainTileLds<kFmt, kMalvar, kSharpen, kTileH>& tl, int tile, int tiles_x,
+                                          int fast_rows, int out_aligned, unsigned long long acc[4]) {
   using G = MainGeom<kFmt, kMalvar, kSharpen, kTileH>;
   constexpr int kR = G::kR;
   constexpr int kH = G::kH;
@@ -653,26 +792,467 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
   constexpr int kTasks = G::kTasks;
   constexpr int kTPT = G::kTPT;
   constexpr int kRW = G::kRW;
-  constexpr int kRH = G::kRH;
   constexpr int kRows3 = G::kRows3;
   constexpr int kLscCols = G::kLscCols;
-  __shared__ __attribute__((aligned(16))) float samp[kSH * kSW];
-  __shared__ __attribute__((aligned(16))) float ring[3][kRH * kRW];
-  __shared__ float lut_s[kLdsLut];
-  __shared__ unsigned int hist[4 * kMaxHistBins];
-  __shared__ __attribute__((aligned(16))) LscTable<kLscCols> lsc_tab;
-  __shared__ float grid_s[kMaxGrid];
-  __shared__ unsigned long long part[kThreads / 64][4];
-
+  const MainCtx& c = cx_;
+  float* samp = tl.samp;
   const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const bool want_hist = f.histogram != nullptr;
-  const bool want_stats = f.stats != nullptr;
-  const int bins = f.bins;
-  const int copies = kMaxHistBins / bins;  // interleaved histogram copies: 4 for 256 bins
-  const int copy = lane & (copies - 1);
   const int W = f.width;
   const int H = f.height;
+  const int bins = c.bins;
+  const int copies = c.copies;
+  const int copy = c.copy;
+  const int tyi = tile / tiles_x;
+  const int x0 = (tile - tyi * tiles_x) * 64;
+  const int y0 = tyi * kTileH;
+  const int sy0 = y0 - kH;  // frame row of sample row 0
+  const int cx0 = x0 - kC;  // frame column of chunk 0
+  const bool interior_x = cx0 >= 0 && cx0 + kNC * kC <= W;
+
+  // 1a. Issue this thread's raw chunk loads.
+  Packed<kFmt> pk[kTPT];
+#pragma unroll
+  for (int t = 0; t < kTPT; ++t) {
+    const int task = tid + t * kThreads;
+    if (task >= kTasks) break;
+    const int sr = task / kNC;
+    const int ci = task - sr * kNC;
+    const int cx = cx0 + ci * kC;
+    if (fast_rows != 0 && cx >= 0 && cx + kC <= W) {
+      const int y = mirror_near(sy0 + sr, H);
+      load_packed<kFmt>(f.raw + static_cast<long long>(y) * f.raw_pitch, cx, pk[t]);
+    }
+  }
+  __syncthreads();  // the previous tile's readers are done
+  int lo = 0;
+  bool table = false;
+  if (c.lsc) {
+    lo = lsc_row(f, sy0 > 0 ? sy0 : 0);
+    table = lsc_row(f, sy0 + kSH - 1 < H - 1 ? sy0 + kSH - 1 : H - 1) - lo <= 1;
+    if (table) build_lsc_table<kLscCols>(f, c.grid, cx0, lo, tl.lsc);
+    __syncthreads();
+  }
+
+  // 1b. Unpack and preprocess (isp_math.hpp:151-155) into the sample tile.
+  unsigned int saturated = 0;
+#pragma unroll
+  for (int t = 0; t < kTPT; ++t) {
+    const int task = tid + t * kThreads;
+    if (task >= kTasks) break;
+    const int sr = task / kNC;
+    const int ci = task - sr * kNC;
+    const int cx = cx0 + ci * kC;
+    const int yv = sy0 + sr;
+    const int y = mirror_near(yv, H);
+    int raw[kC];
+    if (fast_rows != 0 && cx >= 0 && cx + kC <= W) {
+      unpack<kFmt>(pk[t], raw);
+    } else {
+      const std::uint8_t* row = f.raw + static_cast<long long>(y) * f.raw_pitch;
+#pragma unroll
+      for (int k = 0; k < kC; ++k) raw[k] = cx + k >= 0 && cx + k < W ? load_raw(row, kFmt, cx + k) : 0;
+    }
+    const int par = y & 1;
+    const float b0 = par ? c.qc.black[2] : c.qc.black[0];
+    const float b1 = par ? c.qc.black[3] : c.qc.black[1];
+    const float i0 = par ? c.qc.inv[2] : c.qc.inv[0];
+    const float i1 = par ? c.qc.inv[3] : c.qc.inv[1];
+    const float g0 = par ? c.qc.gain[2] : c.qc.gain[0];
+    const float g1 = par ? c.qc.gain[3] : c.qc.gain[1];
+    float tyr = 0.0F;
+    int seg = 0;
+    if (table) seg = lsc_row(f, y, &tyr) - lo;
+    const bool count = c.want_stats && yv >= y0 && yv < y0 + kTileH && yv < H && ci >= 1 && ci <= 64 / kC;
+#pragma unroll
+    for (int h = 0; h < kC / 4; ++h) {
+      const int col0 = kPad - kC + ci * kC + 4 * h;
+      if (col0 < 0 || col0 + 4 > kSW) continue;
+      F4 o;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int k = 4 * h + m;
+        const bool odd = (m & 1) != 0;  // cx and col0 are even
+        float v = (static_cast<float>(raw[k]) - (odd ? b1 : b0)) * (odd ? i1 : i0);
+        v = v > 0.0F ? v : 0.0F;
+        if (c.lsc) {
+          float gain;
+          if (table) {
+            const F2 e = tl.lsc.e[seg][par][ci * kC + k];
+            gain = e.v[0] + e.v[1] * tyr;
+          } else {
+            const int x = cx + k;
+            gain = x >= 0 && x < W ? lsc_gain(f, x, y, cfa_channel(f.cfa, (par << 1) | (x & 1))) : 1.0F;
+          }
+          v = v * gain;
+        }
+        v = v * (odd ? g1 : g0);
+        v = v < 1.0F ? v : 1.0F;
+        o.v[m] = v;
+        if (count && (interior_x || cx + k < W)) saturated += raw[k] >= f.white ? 1U : 0U;
+      }
+      *reinterpret_cast<F4*>(&samp[sr * kSW + col0]) = o;
+    }
+  }
+  acc[3] += saturated;
+  if (!interior_x) {
+    // Columns outside the frame take the sample of their reflection.
+    __syncthreads();
+    for (int i = tid; i < kSH * kSW; i += kThreads) {
+      const int sr = i / kSW;
+      const int col = i - sr * kSW;
+      const int x = x0 - kPad + col;
+      if (x >= 0 && x < W) continue;
+      const int src = mirror_near(x, W) - x0 + kPad;
+      samp[i] = src >= 0 && src < kSW ? samp[sr * kSW + src] : 0.0F;
+    }
+  }
+  __syncthreads();
+
+  // 2. With sharpening: the demosaiced RGB of the tile and a one-pixel ring
+  // (ring column rc = x - x0 + 2, row rr = y - y0 + 1), 8 x 2 pixels per thread.
+  if constexpr (kSharpen) {
+    if (tid < G::kRingBlocks) {
+      const int by = tid / (kRW / 8);
+      const int bx = tid - by * (kRW / 8);
+      const int rc0 = 8 * bx;
+      const int rr0 = 2 * by;
+      // window: sample rows rr0 .. rr0 + 1 + 2R, columns rc0 .. rc0 + 11
+      constexpr int kWR = 2 + 2 * kR;
+      float w[kWR][12];
+#pragma unroll
+      for (int r = 0; r < kWR; ++r) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const F4 a = *reinterpret_cast<const F4*>(&samp[(rr0 + r) * kSW + rc0 + 4 * q]);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        float rgbs[8][3];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          // ring pixel parity: x = x0 - 2 + rc0 + i, y = y0 - 1 + rr0 + j
+          const int ch = cfa_channel(f.cfa, (((1 + j) & 1) << 1) | (i & 1));
+          const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
+          float first, second;
+          demosaic_pair<kMalvar>(w, kR + j, 2 + i, rb, &first, &second);
+          assign_rgb(ch, w[kR + j][2 + i], first, second, rgbs[i]);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            F4 o;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o.v[i] = rgbs[4 * q + i][k];
+            *reinterpret_cast<F4*>(&tl.ring[k][(rr0 + j) * kRW + rc0 + 4 * q]) = o;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // 3. A 4 x kRows3 block of output pixels per thread.
+  const int px = 4 * (tid & 15);
+  const int py = kRows3 * (tid >> 4);
+  const int xo = x0 + px;
+  float rgb[kRows3][4][3];
+  if constexpr (kSharpen) {
+    // sharpen (isp_math.hpp:219-227) over ring rows py .. py + kRows3 + 1, columns px .. px + 7
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float m[kRows3 + 2][8];
+#pragma unroll
+      for (int r = 0; r < kRows3 + 2; ++r) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const F4 a = *reinterpret_cast<const F4*>(&tl.ring[k][(py + r) * kRW + px + 4 * q]);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) m[r][4 * q + e] = a.v[e];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kRows3; ++j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float cc = m[j + 1][i + 2];
+          const float blur = (4.0F * cc + 2.0F * (m[j][i + 2] + m[j + 2][i + 2] + m[j + 1][i + 1] + m[j + 1][i + 3]) +
+                              (m[j][i + 1] + m[j][i + 3] + m[j + 2][i + 1] + m[j + 2][i + 3])) *
+                             0.0625F;
+          rgb[j][i][k] = clamp01(cc + f.sharpen * (cc - blur));
+        }
+      }
+    }
+  } else {
+    // demosaic from samples rows py + kH - kR .. py + kRows3 - 1 + kH + kR, columns px .. px + 11
+    constexpr int kWR = kRows3 + 2 * kR;
+    float w[kWR][12];
+#pragma unroll
+    for (int r = 0; r < kWR; ++r) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const F4 a = *reinterpret_cast<const F4*>(&samp[(py + kH - kR + r) * kSW + px + 4 * q]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kRows3; ++j) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        // y0 is even; py is even for 32-row tiles and may be odd for 16-row tiles.
+        const int ch = cfa_channel(f.cfa, (((py + j) & 1) << 1) | (i & 1));
+        const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
+        float first, second;
+        demosaic_pair<kMalvar>(w, kR + j, 4 + i, rb, &first, &second);
+        assign_rgb(ch, w[kR + j][4 + i], first, second, rgb[j][i]);
+      }
+    }
+  }
+  const bool full_x = xo + 4 <= W;
+#pragma unroll
+  for (int j = 0; j < kRows3; ++j) {
+    const int yo = y0 + py + j;
+    if (yo >= H) break;
+    // correct_color, to_fixed sums, tone, store, output codes (isp_math.hpp:229-305)
+    float lin[4][3];
+    float t[4][3];
+    int q[4][3];
+    unsigned int sums[3] = {0U, 0U, 0U};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        lin[i][k] = clamp01(f.ccm[3 * k] * rgb[j][i][0] + f.ccm[3 * k + 1] * rgb[j][i][1] +
+                            f.ccm[3 * k + 2] * rgb[j][i][2]);
+      }
+      if (c.want_stats && (full_x || xo + i < W)) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sums[k] += static_cast<unsigned int>(lin[i][k] * 1048576.0F + 0.5F);
+      }
+    }
+    if (c.lut_lds) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) t[i][k] = tone(fl.lut, c.n1, c.lut_top, lin[i][k]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) t[i][k] = tone(f.lut, c.n1, c.lut_top, lin[i][k]);
+      }
+    }
+    if (c.want_stats) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[k] += sums[k];
+    }
+    std::uint8_t* orow = f.out + static_cast<long long>(yo) * f.out_pitch;
+    const bool whole = out_aligned != 0 && full_x;
+    switch (f.out_format) {
+      case ISP_OUT_RGB8: {
+        unsigned int byte[12];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const unsigned int v = static_cast<unsigned char>(t[i][k] * 255.0F + 0.5F);
+            q[i][k] = static_cast<int>(v);
+            byte[3 * i + k] = v;
+          }
+        }
+        if (whole) {
+          W3 o;
+#pragma unroll
+          for (int w = 0; w < 3; ++w) {
+            o.v[w] = byte[4 * w] | (byte[4 * w + 1] << 8) | (byte[4 * w + 2] << 16) | (byte[4 * w + 3] << 24);
+          }
+          *reinterpret_cast<W3*>(orow + 3 * xo) = o;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (xo + i >= W) continue;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) orow[3 * (xo + i) + k] = static_cast<std::uint8_t>(byte[3 * i + k]);
+          }
+        }
+        break;
+      }
+      case ISP_OUT_RGBA8: {
+        unsigned int word[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          word[i] = 0xFF000000U;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const unsigned int v = static_cast<unsigned char>(t[i][k] * 255.0F + 0.5F);
+            q[i][k] = static_cast<int>(v);
+            word[i] |= v << (8 * k);
+          }
+        }
+        if (whole) {
+          W4 o;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) o.v[i] = word[i];
+          *reinterpret_cast<W4*>(orow + 4 * xo) = o;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (xo + i >= W) continue;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) orow[4 * (xo + i) + k] = static_cast<std::uint8_t>(word[i] >> (8 * k));
+          }
+        }
+        break;
+      }
+      case ISP_OUT_RGB16: {
+        unsigned int half[12];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const unsigned int v = static_cast<std::uint16_t>(t[i][k] * 65535.0F + 0.5F);
+            half[3 * i + k] = v;
+            q[i][k] = static_cast<int>((v * static_cast<unsigned int>(bins - 1) + 32767U) / 65535U);
+          }
+        }
+        if (whole) {
+#pragma unroll
+          for (int w = 0; w < 3; ++w) {
+            W2 o;
+            o.v[0] = half[4 * w] | (half[4 * w + 1] << 16);
+            o.v[1] = half[4 * w + 2] | (half[4 * w + 3] << 16);
+            *reinterpret_cast<W2*>(orow + 6 * xo + 8 * w) = o;
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (xo + i >= W) continue;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              orow[6 * (xo + i) + 2 * k] = static_cast<std::uint8_t>(half[3 * i + k] & 0xFFU);
+              orow[6 * (xo + i) + 2 * k + 1] = static_cast<std::uint8_t>(half[3 * i + k] >> 8);
+            }
+          }
+        }
+        break;
+      }
+      default: {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            q[i][k] = static_cast<int>(clamp01(t[i][k]) * static_cast<float>(bins - 1) + 0.5F);
+          }
+        }
+        if (whole) {
+#pragma unroll
+          for (int w = 0; w < 3; ++w) {
+            F4 o;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) o.v[k] = t[(4 * w + k) / 3][(4 * w + k) % 3];
+            *reinterpret_cast<F4*>(orow + 12 * xo + 16 * w) = o;
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (xo + i >= W) continue;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) reinterpret_cast<float*>(orow)[3 * (xo + i) + k] = t[i][k];
+          }
+        }
+        break;
+      }
+    }
+    if (c.want_hist) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (!full_x && xo + i >= W) continue;
+        const int luma = (2126 * q[i][0] + 7152 * q[i][1] + 722 * q[i][2] + 5000) / 10000;
+        atomicAdd(&fl.hist[luma * copies + copy], 1U);
+        atomicAdd(&fl.hist[(bins + q[i][0]) * copies + copy], 1U);
+        atomicAdd(&fl.hist[(2 * bins + q[i][1]) * copies + copy], 1U);
+        atomicAdd(&fl.hist[(3 * bins + q[i][2]) * copies + copy], 1U);
+      }
+    }
+  }
+}
+
+// Adds the block's sums and histograms to the frame's scratch; returns the
+// token of the atomics (see complete()).
+__device__ __forceinline__ unsigned int main_flush(MainScratch* m, MainFrameLds& fl, const MainCtx& c,
+                                                   const unsigned long long acc[4]) {
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  if (c.want_stats) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const unsigned long long v = wave_sum(acc[k]);
+      if ((tid & 63) == 0) fl.part[wave][k] = v;
+    }
+  }
+  __syncthreads();
+  unsigned int token = 0;
+  if (c.want_stats && tid < 4) {
+    unsigned long long total = 0;
+    for (int w = 0; w < kThreads / 64; ++w) total += fl.part[w][tid];
+    if (total != 0) token ^= fold(atomicAdd(&m->sum[tid], total));
+  }
+  if (c.want_hist) {
+    const int bins = c.bins;
+    for (int i = tid; i < 4 * bins; i += kThreads) {
+      unsigned int total = 0;
+      for (int k = 0; k < c.copies; ++k) total += fl.hist[i * c.copies + k];
+      if (total != 0U) token ^= atomicAdd(&m->hist[i / bins][i % bins], total);
+    }
+  }
+  return token;
+}
+
+// The frame's last contributor: the histograms and the statistics vector
+// (finish_stats, isp_math.hpp:387-425); the reads clear the scratch. With a
+// device exposure state, the frame runs at the state's exposure and leaves its
+// suggestion there for the stream's next frame.
+__device__ __forceinline__ void main_finish(const FrameDesc& f, MainScratch* m, MainFrameLds& fl, const MainCtx& c) {
+  const int tid = threadIdx.x;
+  if (c.want_hist) {
+    const int bins = c.bins;
+    for (int i = tid; i < 4 * bins; i += kThreads) f.histogram[i] = atomicExch(&m->hist[i / bins][i % bins], 0U);
+  }
+  if (tid < 4) fl.sums[tid] = atomicExch(&m->sum[tid], 0ULL);
+  __syncthreads();
+  if (c.want_stats && tid == 0) {
+    const unsigned long long sums[4] = {fl.sums[0], fl.sums[1], fl.sums[2], fl.sums[3]};
+    long long exposure_ns = f.exposure_ns;
+    float exposure_gain = f.exposure_gain;
+    AeState* ae = f.ae_state;
+    if (ae != nullptr && ae->valid != 0) {
+      exposure_ns = ae->exposure_ns;
+      exposure_gain = ae->gain;
+    }
+    float out[ISP_STATS_SIZE];
+    finish_stats(sums, static_cast<long long>(f.width) * f.height, c.gains, f.ae, exposure_ns, exposure_gain, out);
+    if (f.stats != nullptr) {
+      for (int k = 0; k < ISP_STATS_SIZE; ++k) f.stats[k] = out[k];
+    }
+    if (ae != nullptr && out[ISP_STAT_SUGGESTED_EXPOSURE_NS] > 0.0F) {
+      ae->exposure_ns = static_cast<long long>(static_cast<double>(out[ISP_STAT_SUGGESTED_EXPOSURE_NS]) + 0.5);
+      ae->gain = out[ISP_STAT_SUGGESTED_GAIN];
+      ae->valid = 1;
+    }
+  }
+  if (tid == 0) atomicExch(&m->done, 0U);
+}
+
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
+__global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles,
+                                                        int fast_rows, int out_aligned) {
+  __shared__ MainFrameLds fl;
+  __shared__ MainTileLds<kFmt, kMalvar, kSharpen, kTileH> tl;
   float gains[3];
   if (f.wb_mode != ISP_WB_MANUAL) {
 #pragma unroll
@@ -681,440 +1261,221 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
 #pragma unroll
     for (int k = 0; k < 3; ++k) gains[k] = f.wb[k];
   }
-  const QuadConst qc = quad_consts(f, gains);
-  if (want_hist) {
-    for (int i = tid; i < 4 * kMaxHistBins; i += kThreads) hist[i] = 0U;
-  }
-  const bool lut_lds = f.lut_size <= kLdsLut;
-  if (lut_lds) {
-    for (int i = tid; i < f.lut_size; i += kThreads) lut_s[i] = f.lut[i];
-  }
-  const int n1 = f.lut_size - 1;
-  const float lut_top = f.lut[n1];
-  const bool lsc = f.lsc != nullptr;
-  const float* grid = stage_grid(f, grid_s);
+  MainCtx c;
+  main_begin(f, gains, fl, c);
   unsigned long long acc[4] = {0, 0, 0, 0};
-
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const int tyi = tile / tiles_x;
-    const int x0 = (tile - tyi * tiles_x) * 64;
-    const int y0 = tyi * kTileH;
-    const int sy0 = y0 - kH;  // frame row of sample row 0
-    const int cx0 = x0 - kC;  // frame column of chunk 0
-    const bool interior_x = cx0 >= 0 && cx0 + kNC * kC <= W;
+    main_tile<kFmt, kMalvar, kSharpen, kTileH>(f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
+  }
+  const unsigned int token = main_flush(&s->main, fl, c, acc);
+  if (!complete(&s->main.done, 1U, gridDim.x, token, &fl.flag)) return;
+  main_finish(f, &s->main, fl, c);
+}
 
-    // 1a. Issue this thread's raw chunk loads.
-    Packed<kFmt> pk[kTPT];
-#pragma unroll
-    for (int t = 0; t < kTPT; ++t) {
-      const int task = tid + t * kThreads;
-      if (task >= kTasks) break;
-      const int sr = task / kNC;
-      const int c = task - sr * kNC;
-      const int cx = cx0 + c * kC;
-      if (fast_rows != 0 && cx >= 0 && cx + kC <= W) {
-        const int y = mirror_near(sy0 + sr, H);
-        load_packed<kFmt>(f.raw + static_cast<long long>(y) * f.raw_pitch, cx, pk[t]);
+// ---- the batched form ----------------------------------------------------------
+
+template <typename A, typename... B>
+constexpr std::size_t max_size() {
+  if constexpr (sizeof...(B) == 0) {
+    return sizeof(A);
+  } else {
+    return sizeof(A) > max_size<B...>() ? sizeof(A) : max_size<B...>();
+  }
+}
+
+// The largest tile LDS of every body the batch kernel runs (sharpening with
+// 32-row tiles dominates; RAW10's 16-sample chunks widen its lens shading table).
+constexpr std::size_t kBatchTileLds =
+    max_size<AwbTileLds, MainTileLds<ISP_RAW_U8, true, true, 32>, MainTileLds<ISP_RAW_U16, true, true, 32>,
+             MainTileLds<ISP_RAW_RAW10, true, true, 32>, MainTileLds<ISP_RAW_RAW12, true, true, 32>,
+             MainTileLds<ISP_RAW_RAW10, false, true, 32>, MainTileLds<ISP_RAW_RAW12, false, true, 32>>();
+constexpr std::size_t kBatchFrameLds = max_size<AwbFrameLds, MainFrameLds>();
+
+constexpr int variant_of(int fmt, bool malvar, bool sharpen, int tile_h) {
+  return fmt | (malvar ? 4 : 0) | (sharpen ? 8 : 0) | (tile_h == 16 ? 16 : 0);
+}
+
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
+__device__ __forceinline__ void batch_main_item(const BatchFrame& b, const MainCtx& c, MainFrameLds& fl,
+                                                unsigned char* tile_lds, int item, unsigned long long acc[4]) {
+  auto& tl = *reinterpret_cast<MainTileLds<kFmt, kMalvar, kSharpen, kTileH>*>(tile_lds);
+  const int first = item * b.tiles_per_item;
+  const int end = min(first + b.tiles_per_item, b.tiles);
+  for (int t = first; t < end; ++t) {
+    main_tile<kFmt, kMalvar, kSharpen, kTileH>(b.f, c, fl, tl, t, b.tiles_x, b.fast_rows, b.out_aligned, acc);
+  }
+}
+
+template <int kFmt>
+__device__ __forceinline__ void batch_awb_item(const BatchFrame& b, const AwbCtx& c, AwbFrameLds& fl,
+                                               unsigned char* tile_lds, int item, unsigned long long acc[4]) {
+  auto& tl = *reinterpret_cast<AwbTileLds*>(tile_lds);
+  const int first = item * b.awb_tiles_per_item;
+  const int end = min(first + b.awb_tiles_per_item, b.awb_tiles);
+  for (int t = first; t < end; ++t) awb_tile<kFmt>(b.f, c, fl, tl, t, b.awb_tiles_x, b.fast_rows, acc);
+}
+
+#define ISP_BATCH_MAIN_CASE(FMT, MAL, SHP, TH)                                             \
+  case variant_of(FMT, MAL, SHP, TH):                                                      \
+    batch_main_item<FMT, MAL, SHP, TH>(b, mc, mfl, tile_lds, local, acc);                  \
+    break;
+#define ISP_BATCH_MAIN_CASES(FMT)                                                         \
+  ISP_BATCH_MAIN_CASE(FMT, false, false, 32)                                              \
+  ISP_BATCH_MAIN_CASE(FMT, false, true, 32)                                               \
+  ISP_BATCH_MAIN_CASE(FMT, true, false, 32)                                               \
+  ISP_BATCH_MAIN_CASE(FMT, true, true, 32)                                                \
+  ISP_BATCH_MAIN_CASE(FMT, false, false, 16)                                              \
+  ISP_BATCH_MAIN_CASE(FMT, false, true, 16)                                               \
+  ISP_BATCH_MAIN_CASE(FMT, true, false, 16)                                               \
+  ISP_BATCH_MAIN_CASE(FMT, true, true, 16)
+
+// The frame (in the pass's order) that holds queue item `local` of a pass, by one
+// coalesced load of the pass's start table per wave; returns its index in the
+// table and the item within the frame.
+__device__ __forceinline__ int batch_locate(const int* start, const int* order, int count, int local, int* within) {
+  const int lane = threadIdx.x & 63;
+  const int s = lane < count ? start[lane] : INT_MAX;
+  const unsigned long long mask = __ballot(s <= local);
+  const int k = 63 - __clzll(static_cast<long long>(mask));
+  const int ks = __builtin_amdgcn_readfirstlane(k);
+  *within = local - start[ks];
+  return order[ks];
+}
+
+__global__ void __launch_bounds__(kThreads) batch_kernel(const BatchTable* __restrict__ table, BatchControl* ctl,
+                                                         unsigned int spin_limit) {
+  __shared__ __attribute__((aligned(16))) unsigned char frame_lds[kBatchFrameLds];
+  __shared__ __attribute__((aligned(16))) unsigned char tile_lds[kBatchTileLds];
+  __shared__ int slot[2];  // the next work item; the launch's tick
+  auto& afl = *reinterpret_cast<AwbFrameLds*>(frame_lds);
+  auto& mfl = *reinterpret_cast<MainFrameLds*>(frame_lds);
+  const int tid = threadIdx.x;
+  const int count = table->count;
+  const int awb_items = table->awb_items;
+  const int total = table->total_items;
+  unsigned int pending = 0;
+  if (tid == 0) {
+    slot[1] = static_cast<int>(atomicOr(&ctl->tick, 0U));
+    slot[0] = static_cast<int>(atomicAdd(&ctl->next, 1U));
+  }
+  int cur = -1;    // frame of the open contribution
+  int cur_pass = 0;  // its pass: 0 white balance, 1 main
+  unsigned int items_done = 0;
+  unsigned long long acc[4] = {0, 0, 0, 0};
+  AwbCtx ac;
+  MainCtx mc;
+  __syncthreads();
+  const unsigned int ready_value = static_cast<unsigned int>(slot[1]) + 1U;
+
+  // Closes the block's contribution to frame `cur`; the last contributor runs
+  // the frame's epilogue.
+  auto close = [&]() {
+    const BatchFrame& b = table->frames[cur];
+    if (cur_pass == 0) {
+      const unsigned int token = awb_flush(&b.s->awb, afl, ac, acc);
+      if (complete(&b.s->awb.done, items_done, static_cast<unsigned int>(b.awb_items), token, &afl.flag)) {
+        awb_finish(b.f, b.s, afl, ac, &ctl->ready[cur], ready_value);
+      }
+    } else {
+      const unsigned int token = main_flush(&b.s->main, mfl, mc, acc);
+      if (complete(&b.s->main.done, items_done, static_cast<unsigned int>(b.items), token, &mfl.flag)) {
+        main_finish(b.f, &b.s->main, mfl, mc);
       }
     }
-    __syncthreads();  // the previous tile's readers are done
-    int lo = 0;
-    bool table = false;
-    if (lsc) {
-      lo = lsc_row(f, sy0 > 0 ? sy0 : 0);
-      table = lsc_row(f, sy0 + kSH - 1 < H - 1 ? sy0 + kSH - 1 : H - 1) - lo <= 1;
-      if (table) build_lsc_table<kLscCols>(f, grid, cx0, lo, lsc_tab);
-      __syncthreads();
-    }
-
-    // 1b. Unpack and preprocess (isp_math.hpp:151-155) into the sample tile.
-    unsigned int saturated = 0;
+    __syncthreads();  // the epilogue is done with the frame LDS
 #pragma unroll
-    for (int t = 0; t < kTPT; ++t) {
-      const int task = tid + t * kThreads;
-      if (task >= kTasks) break;
-      const int sr = task / kNC;
-      const int c = task - sr * kNC;
-      const int cx = cx0 + c * kC;
-      const int yv = sy0 + sr;
-      const int y = mirror_near(yv, H);
-      int raw[kC];
-      if (fast_rows != 0 && cx >= 0 && cx + kC <= W) {
-        unpack<kFmt>(pk[t], raw);
+    for (int k = 0; k < 4; ++k) acc[k] = 0;
+    items_done = 0;
+  };
+
+  for (;;) {
+    const int item = slot[0];
+    __syncthreads();  // every thread has the item before thread 0 replaces it
+    if (item >= total) break;
+    if (tid == 0) pending = atomicAdd(&ctl->next, 1U);  // the next item, in flight meanwhile
+    const int pass = item < awb_items ? 0 : 1;
+    int local = 0;
+    const int f = pass == 0 ? batch_locate(table->awb_start, table->awb_order, count, item, &local)
+                            : batch_locate(table->main_start, table->main_order, count, item - awb_items, &local);
+    if (f != cur || pass != cur_pass) {
+      if (cur >= 0) close();
+      cur = f;
+      cur_pass = pass;
+      const BatchFrame& b = table->frames[f];
+      if (pass == 0) {
+        awb_begin(b.f, afl, ac);
       } else {
-        const std::uint8_t* row = f.raw + static_cast<long long>(y) * f.raw_pitch;
-#pragma unroll
-        for (int k = 0; k < kC; ++k) raw[k] = cx + k >= 0 && cx + k < W ? load_raw(row, kFmt, cx + k) : 0;
-      }
-      const int par = y & 1;
-      const float b0 = par ? qc.black[2] : qc.black[0];
-      const float b1 = par ? qc.black[3] : qc.black[1];
-      const float i0 = par ? qc.inv[2] : qc.inv[0];
-      const float i1 = par ? qc.inv[3] : qc.inv[1];
-      const float g0 = par ? qc.gain[2] : qc.gain[0];
-      const float g1 = par ? qc.gain[3] : qc.gain[1];
-      float tyr = 0.0F;
-      int seg = 0;
-      if (table) seg = lsc_row(f, y, &tyr) - lo;
-      const bool count = want_stats && yv >= y0 && yv < y0 + kTileH && yv < H && c >= 1 && c <= 64 / kC;
-#pragma unroll
-      for (int h = 0; h < kC / 4; ++h) {
-        const int col0 = kPad - kC + c * kC + 4 * h;
-        if (col0 < 0 || col0 + 4 > kSW) continue;
-        F4 o;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          const int k = 4 * h + m;
-          const bool odd = (m & 1) != 0;  // cx and col0 are even
-          float v = (static_cast<float>(raw[k]) - (odd ? b1 : b0)) * (odd ? i1 : i0);
-          v = v > 0.0F ? v : 0.0F;
-          if (lsc) {
-            float gain;
-            if (table) {
-              const F2 e = lsc_tab.e[seg][par][c * kC + k];
-              gain = e.v[0] + e.v[1] * tyr;
-            } else {
-              const int x = cx + k;
-              gain = x >= 0 && x < W ? lsc_gain(f, x, y, cfa_channel(f.cfa, (par << 1) | (x & 1))) : 1.0F;
+        float gains[3];
+        if (b.f.wb_mode != ISP_WB_MANUAL) {
+          if (tid == 0) {
+            // Bounded wait for this frame's white balance (see the file comment).
+            unsigned int rounds = 0;
+            while (atomicOr(&ctl->ready[f], 0U) != ready_value) {
+              if (++rounds > spin_limit) {
+                atomicOr(&ctl->error, 1U);
+                break;
+              }
+              __builtin_amdgcn_s_sleep(8);
             }
-            v = v * gain;
+            for (int k = 0; k < 3; ++k) mfl.gains[k] = coherent_read(&b.s->wb.gains[k]);
           }
-          v = v * (odd ? g1 : g0);
-          v = v < 1.0F ? v : 1.0F;
-          o.v[m] = v;
-          if (count && (interior_x || cx + k < W)) saturated += raw[k] >= f.white ? 1U : 0U;
-        }
-        *reinterpret_cast<F4*>(&samp[sr * kSW + col0]) = o;
-      }
-    }
-    acc[3] += saturated;
-    if (!interior_x) {
-      // Columns outside the frame take the sample of their reflection.
-      __syncthreads();
-      for (int i = tid; i < kSH * kSW; i += kThreads) {
-        const int sr = i / kSW;
-        const int col = i - sr * kSW;
-        const int x = x0 - kPad + col;
-        if (x >= 0 && x < W) continue;
-        const int src = mirror_near(x, W) - x0 + kPad;
-        samp[i] = src >= 0 && src < kSW ? samp[sr * kSW + src] : 0.0F;
-      }
-    }
-    __syncthreads();
-
-    // 2. With sharpening: the demosaiced RGB of the tile and a one-pixel ring
-    // (ring column rc = x - x0 + 2, row rr = y - y0 + 1), 8 x 2 pixels per thread.
-    if constexpr (kSharpen) {
-      if (tid < G::kRingBlocks) {
-        const int by = tid / (kRW / 8);
-        const int bx = tid - by * (kRW / 8);
-        const int rc0 = 8 * bx;
-        const int rr0 = 2 * by;
-        // window: sample rows rr0 .. rr0 + 1 + 2R, columns rc0 .. rc0 + 11
-        constexpr int kWR = 2 + 2 * kR;
-        float w[kWR][12];
+          __syncthreads();
 #pragma unroll
-        for (int r = 0; r < kWR; ++r) {
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            const F4 a = *reinterpret_cast<const F4*>(&samp[(rr0 + r) * kSW + rc0 + 4 * q]);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          float rgbs[8][3];
-#pragma unroll
-          for (int i = 0; i < 8; ++i) {
-            // ring pixel parity: x = x0 - 2 + rc0 + i, y = y0 - 1 + rr0 + j
-            const int ch = cfa_channel(f.cfa, (((1 + j) & 1) << 1) | (i & 1));
-            const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
-            float first, second;
-            demosaic_pair<kMalvar>(w, kR + j, 2 + i, rb, &first, &second);
-            assign_rgb(ch, w[kR + j][2 + i], first, second, rgbs[i]);
-          }
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-              F4 o;
-#pragma unroll
-              for (int i = 0; i < 4; ++i) o.v[i] = rgbs[4 * q + i][k];
-              *reinterpret_cast<F4*>(&ring[k][(rr0 + j) * kRW + rc0 + 4 * q]) = o;
-            }
-          }
-        }
-      }
-      __syncthreads();
-    }
-
-    // 3. A 4 x kRows3 block of output pixels per thread.
-    {
-      const int px = 4 * (tid & 15);
-      const int py = kRows3 * (tid >> 4);
-      const int xo = x0 + px;
-      float rgb[kRows3][4][3];
-      if constexpr (kSharpen) {
-        // sharpen (isp_math.hpp:219-227) over ring rows py .. py + kRows3 + 1, columns px .. px + 7
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          float m[kRows3 + 2][8];
-#pragma unroll
-          for (int r = 0; r < kRows3 + 2; ++r) {
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-              const F4 a = *reinterpret_cast<const F4*>(&ring[k][(py + r) * kRW + px + 4 * q]);
-#pragma unroll
-              for (int e = 0; e < 4; ++e) m[r][4 * q + e] = a.v[e];
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < kRows3; ++j) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const float c = m[j + 1][i + 2];
-              const float blur = (4.0F * c + 2.0F * (m[j][i + 2] + m[j + 2][i + 2] + m[j + 1][i + 1] + m[j + 1][i + 3]) +
-                                  (m[j][i + 1] + m[j][i + 3] + m[j + 2][i + 1] + m[j + 2][i + 3])) *
-                                 0.0625F;
-              rgb[j][i][k] = clamp01(c + f.sharpen * (c - blur));
-            }
-          }
-        }
-      } else {
-        // demosaic from samples rows py + kH - kR .. py + kRows3 - 1 + kH + kR, columns px .. px + 11
-        constexpr int kWR = kRows3 + 2 * kR;
-        float w[kWR][12];
-#pragma unroll
-        for (int r = 0; r < kWR; ++r) {
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            const F4 a = *reinterpret_cast<const F4*>(&samp[(py + kH - kR + r) * kSW + px + 4 * q]);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < kRows3; ++j) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int ch = cfa_channel(f.cfa, ((j & 1) << 1) | (i & 1));
-            const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
-            float first, second;
-            demosaic_pair<kMalvar>(w, kR + j, 4 + i, rb, &first, &second);
-            assign_rgb(ch, w[kR + j][4 + i], first, second, rgb[j][i]);
-          }
-        }
-      }
-      const bool full_x = xo + 4 <= W;
-#pragma unroll
-      for (int j = 0; j < kRows3; ++j) {
-        const int yo = y0 + py + j;
-        if (yo >= H) break;
-        // correct_color, to_fixed sums, tone, store, output codes (isp_math.hpp:229-305)
-        float lin[4][3];
-        float t[4][3];
-        int q[4][3];
-        unsigned int sums[3] = {0U, 0U, 0U};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            lin[i][k] = clamp01(f.ccm[3 * k] * rgb[j][i][0] + f.ccm[3 * k + 1] * rgb[j][i][1] +
-                                f.ccm[3 * k + 2] * rgb[j][i][2]);
-          }
-          if (want_stats && (full_x || xo + i < W)) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) sums[k] += static_cast<unsigned int>(lin[i][k] * 1048576.0F + 0.5F);
-          }
-        }
-        if (lut_lds) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) t[i][k] = tone(lut_s, n1, lut_top, lin[i][k]);
-          }
+          for (int k = 0; k < 3; ++k) gains[k] = mfl.gains[k];
         } else {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) t[i][k] = tone(f.lut, n1, lut_top, lin[i][k]);
-          }
+          for (int k = 0; k < 3; ++k) gains[k] = b.f.wb[k];
         }
-        if (want_stats) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) acc[k] += sums[k];
-        }
-        std::uint8_t* orow = f.out + static_cast<long long>(yo) * f.out_pitch;
-        const bool whole = out_aligned != 0 && full_x;
-        switch (f.out_format) {
-          case ISP_OUT_RGB8: {
-            unsigned int byte[12];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-#pragma unroll
-              for (int k = 0; k < 3; ++k) {
-                const unsigned int v = static_cast<unsigned char>(t[i][k] * 255.0F + 0.5F);
-                q[i][k] = static_cast<int>(v);
-                byte[3 * i + k] = v;
-              }
-            }
-            if (whole) {
-              W3 o;
-#pragma unroll
-              for (int w = 0; w < 3; ++w) {
-                o.v[w] = byte[4 * w] | (byte[4 * w + 1] << 8) | (byte[4 * w + 2] << 16) | (byte[4 * w + 3] << 24);
-              }
-              *reinterpret_cast<W3*>(orow + 3 * xo) = o;
-            } else {
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                if (xo + i >= W) continue;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) orow[3 * (xo + i) + k] = static_cast<std::uint8_t>(byte[3 * i + k]);
-              }
-            }
-            break;
-          }
-          case ISP_OUT_RGBA8: {
-            unsigned int word[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              word[i] = 0xFF000000U;
-#pragma unroll
-              for (int k = 0; k < 3; ++k) {
-                const unsigned int v = static_cast<unsigned char>(t[i][k] * 255.0F + 0.5F);
-                q[i][k] = static_cast<int>(v);
-                word[i] |= v << (8 * k);
-              }
-            }
-            if (whole) {
-              W4 o;
-#pragma unroll
-              for (int i = 0; i < 4; ++i) o.v[i] = word[i];
-              *reinterpret_cast<W4*>(orow + 4 * xo) = o;
-            } else {
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                if (xo + i >= W) continue;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) orow[4 * (xo + i) + k] = static_cast<std::uint8_t>(word[i] >> (8 * k));
-              }
-            }
-            break;
-          }
-          case ISP_OUT_RGB16: {
-            unsigned int half[12];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-#pragma unroll
-              for (int k = 0; k < 3; ++k) {
-                const unsigned int v = static_cast<std::uint16_t>(t[i][k] * 65535.0F + 0.5F);
-                half[3 * i + k] = v;
-                q[i][k] = static_cast<int>((v * static_cast<unsigned int>(bins - 1) + 32767U) / 65535U);
-              }
-            }
-            if (whole) {
-#pragma unroll
-              for (int w = 0; w < 3; ++w) {
-                W2 o;
-                o.v[0] = half[4 * w] | (half[4 * w + 1] << 16);
-                o.v[1] = half[4 * w + 2] | (half[4 * w + 3] << 16);
-                *reinterpret_cast<W2*>(orow + 6 * xo + 8 * w) = o;
-              }
-            } else {
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                if (xo + i >= W) continue;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                  orow[6 * (xo + i) + 2 * k] = static_cast<std::uint8_t>(half[3 * i + k] & 0xFFU);
-                  orow[6 * (xo + i) + 2 * k + 1] = static_cast<std::uint8_t>(half[3 * i + k] >> 8);
-                }
-              }
-            }
-            break;
-          }
-          default: {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-#pragma unroll
-              for (int k = 0; k < 3; ++k) {
-                q[i][k] = static_cast<int>(clamp01(t[i][k]) * static_cast<float>(bins - 1) + 0.5F);
-              }
-            }
-            if (whole) {
-#pragma unroll
-              for (int w = 0; w < 3; ++w) {
-                F4 o;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) o.v[k] = t[(4 * w + k) / 3][(4 * w + k) % 3];
-                *reinterpret_cast<F4*>(orow + 12 * xo + 16 * w) = o;
-              }
-            } else {
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                if (xo + i >= W) continue;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) reinterpret_cast<float*>(orow)[3 * (xo + i) + k] = t[i][k];
-              }
-            }
-            break;
-          }
-        }
-        if (want_hist) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            if (!full_x && xo + i >= W) continue;
-            const int luma = (2126 * q[i][0] + 7152 * q[i][1] + 722 * q[i][2] + 5000) / 10000;
-            atomicAdd(&hist[luma * copies + copy], 1U);
-            atomicAdd(&hist[(bins + q[i][0]) * copies + copy], 1U);
-            atomicAdd(&hist[(2 * bins + q[i][1]) * copies + copy], 1U);
-            atomicAdd(&hist[(3 * bins + q[i][2]) * copies + copy], 1U);
-          }
-        }
+        main_begin(b.f, gains, mfl, mc);
       }
     }
+    const BatchFrame& b = table->frames[f];
+    if (pass == 0) {
+      switch (b.f.raw_format) {
+        case ISP_RAW_U8:
+          batch_awb_item<ISP_RAW_U8>(b, ac, afl, tile_lds, local, acc);
+          break;
+        case ISP_RAW_U16:
+          batch_awb_item<ISP_RAW_U16>(b, ac, afl, tile_lds, local, acc);
+          break;
+        case ISP_RAW_RAW10:
+          batch_awb_item<ISP_RAW_RAW10>(b, ac, afl, tile_lds, local, acc);
+          break;
+        default:
+          batch_awb_item<ISP_RAW_RAW12>(b, ac, afl, tile_lds, local, acc);
+          break;
+      }
+    } else {
+      switch (b.variant) {
+        ISP_BATCH_MAIN_CASES(ISP_RAW_U8)
+        ISP_BATCH_MAIN_CASES(ISP_RAW_U16)
+        ISP_BATCH_MAIN_CASES(ISP_RAW_RAW10)
+        ISP_BATCH_MAIN_CASES(ISP_RAW_RAW12)
+        default:
+          break;
+      }
+    }
+    ++items_done;
+    if (tid == 0) slot[0] = static_cast<int>(pending);
+    __syncthreads();
   }
+  if (cur >= 0) close();
 
-  // Flush: the block's sums and histograms into the frame's scratch.
-  const int wave = tid >> 6;
-  if (want_stats) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const unsigned long long v = wave_sum(acc[k]);
-      if ((tid & 63) == 0) part[wave][k] = v;
+  // The last block out leaves the queue ready for the next launch.
+  if (tid == 0) {
+    const unsigned int ticket = atomicAdd(&ctl->exited, 1U);
+    if (ticket == gridDim.x - 1) {
+      atomicExch(&ctl->next, 0U);
+      atomicExch(&ctl->exited, 0U);
+      atomicAdd(&ctl->tick, 1U);
     }
   }
-  __syncthreads();
-  unsigned int token = 0;
-  if (want_stats && tid < 4) {
-    unsigned long long total = 0;
-    for (int w = 0; w < kThreads / 64; ++w) total += part[w][tid];
-    if (total != 0) token ^= fold(atomicAdd(&s->main.sum[tid], total));
-  }
-  if (want_hist) {
-    for (int i = tid; i < 4 * bins; i += kThreads) {
-      unsigned int total = 0;
-      for (int c = 0; c < copies; ++c) total += hist[i * copies + c];
-      if (total != 0U) token ^= atomicAdd(&s->main.hist[i / bins][i % bins], total);
-    }
-  }
-  if (!last_block(&s->main.done, token)) return;
-
-  // The last block: the histograms and the statistics vector; the reads clear.
-  if (want_hist) {
-    for (int i = tid; i < 4 * bins; i += kThreads) f.histogram[i] = atomicExch(&s->main.hist[i / bins][i % bins], 0U);
-  }
-  __shared__ unsigned long long sums_s[4];
-  if (tid < 4) sums_s[tid] = atomicExch(&s->main.sum[tid], 0ULL);
-  __syncthreads();
-  if (want_stats && tid == 0) {
-    const unsigned long long sums[4] = {sums_s[0], sums_s[1], sums_s[2], sums_s[3]};
-    finish_stats(sums, static_cast<long long>(W) * H, gains, f.ae, f.exposure_ns, f.exposure_gain, f.stats);
-  }
-  if (tid == 0) atomicExch(&s->main.done, 0U);
 }
+
+#undef ISP_BATCH_MAIN_CASES
+#undef ISP_BATCH_MAIN_CASE
+
+// ---- launches ---------------------------------------------------------------
 
 void check(hipError_t status, const char* what) {
   if (status != hipSuccess) {
@@ -1220,6 +1581,20 @@ void launch_main(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen
       launch_main_f<ISP_RAW_RAW12>(f, s, malvar, sharpen, cfg, st);
       break;
   }
+}
+
+namespace {
+std::atomic<unsigned int> g_spin_limit{1U << 24};  // ~ 8 * 64 clocks a round: about 4 s at 2.4 GHz
+}  // namespace
+
+void set_batch_spin_limit(unsigned int rounds) { g_spin_limit = rounds; }
+
+int batch_blocks_per_cu() { return blocks_per_cu(batch_kernel); }
+
+void launch_batch(const BatchTable* table, BatchControl* control, int grid, void* stream) {
+  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kThreads), 0, static_cast<hipStream_t>(stream), table, control,
+                     g_spin_limit.load());
+  check(hipGetLastError(), "batch_kernel launch");
 }
 
 }  // namespace isp_amd

@@ -7,164 +7,10 @@
 // partials, completion counters) and one event. Frames share the scratch and
 // the white balance state, so each frame's GPU work waits for the previous
 // frame's (hipStreamWaitEvent, never a host wait) and frames run in call order.
-#include <hip/hip_runtime.h>
-
-#include <cmath>
-#include <cstring>
-#include <memory>
-#include <string>
-
-#include "isp_internal.hpp"
+#include "isp_pipeline.hpp"
 
 using namespace isp_amd;
 
-namespace {
-
-void check(hipError_t status, const char* what) {
-  if (status != hipSuccess) {
-    throw runtime_failure(std::string("IspPipeline (hip): ") + what + ": " + hipGetErrorString(status));
-  }
-}
-
-// kCfa (reference cpu_pipeline.cpp:15-20): channel at quad position (y & 1) * 2 + (x & 1).
-constexpr int kCfa[4][4] = {
-    {ISP_CHANNEL_R, ISP_CHANNEL_GR, ISP_CHANNEL_GB, ISP_CHANNEL_B},
-    {ISP_CHANNEL_B, ISP_CHANNEL_GB, ISP_CHANNEL_GR, ISP_CHANNEL_R},
-    {ISP_CHANNEL_GR, ISP_CHANNEL_R, ISP_CHANNEL_B, ISP_CHANNEL_GB},
-    {ISP_CHANNEL_GB, ISP_CHANNEL_B, ISP_CHANNEL_R, ISP_CHANNEL_GR},
-};
-
-template <typename F>
-isp_status guarded(F&& body) {
-  try {
-    body();
-    clear_error();
-    return ISP_OK;
-  } catch (const invalid_argument& e) {
-    set_error(e.what());
-    return ISP_INVALID_ARGUMENT;
-  } catch (const std::bad_alloc&) {
-    set_error("IspPipeline (hip): out of host memory");
-    return ISP_FAILURE;
-  } catch (const std::exception& e) {
-    set_error(e.what());
-    return ISP_FAILURE;
-  }
-}
-
-// Copies host memory to the device on a stream of its own and waits for it
-// (the frames' streams would not otherwise wait for a pageable copy).
-void upload(const void* source, std::size_t bytes, void* destination) {
-  hipStream_t stream = nullptr;
-  check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-  const hipError_t copied = hipMemcpyAsync(destination, source, bytes, hipMemcpyHostToDevice, stream);
-  const hipError_t done = copied == hipSuccess ? hipStreamSynchronize(stream) : copied;
-  static_cast<void>(hipStreamDestroy(stream));
-  check(done, "upload");
-}
-
-}  // namespace
-
-struct isp_pipeline {
-  Params params;
-  std::vector<float> lut;
-  int device = 0;
-  LaunchConfig launch{};
-  float* d_lut = nullptr;
-  float* d_lsc = nullptr;
-  DeviceScratch* d_scratch = nullptr;
-  hipEvent_t done = nullptr;
-  bool queued = false;
-  // Optional kernel timing: events around each launch, on the launch stream.
-  bool timing = false;
-  std::vector<hipEvent_t> marks;  // awb begin, main begin, main end, per frame
-  std::vector<int> mark_kind;
-  std::size_t next_mark = 0;
-
-  ~isp_pipeline() { release(); }
-
-  hipEvent_t mark(hipStream_t stream) {
-    if (next_mark == marks.size()) {
-      hipEvent_t e = nullptr;
-      check(hipEventCreate(&e), "hipEventCreate");
-      marks.push_back(e);
-    }
-    hipEvent_t e = marks[next_mark++];
-    check(hipEventRecord(e, stream), "hipEventRecord");
-    return e;
-  }
-
-  void release() noexcept {
-    if (queued && done != nullptr) static_cast<void>(hipEventSynchronize(done));
-    if (d_lut != nullptr) static_cast<void>(hipFree(d_lut));
-    if (d_lsc != nullptr) static_cast<void>(hipFree(d_lsc));
-    if (d_scratch != nullptr) static_cast<void>(hipFree(d_scratch));
-    if (done != nullptr) static_cast<void>(hipEventDestroy(done));
-    for (hipEvent_t e : marks) static_cast<void>(hipEventDestroy(e));
-    marks.clear();
-    d_lut = d_lsc = nullptr;
-    d_scratch = nullptr;
-    done = nullptr;
-    queued = false;
-  }
-
-  // initial_white_balance (cpu_pipeline.cpp:90-95) into the device state.
-  void reset() {
-    if (queued) check(hipEventSynchronize(done), "hipEventSynchronize");
-    WbState state{};
-    for (int k = 0; k < 3; ++k) state.gains[k] = params.c.wb_gains[k];
-    state.valid = 0;
-    upload(&state, sizeof(state), &d_scratch->wb);
-  }
-
-  // make_frame_params (cpu_pipeline.cpp:50-88) with the device tables.
-  FrameDesc frame(const isp_raw_view& raw, const isp_rgb_view& out, const isp_stats_out* stats,
-                  const isp_frame_exposure* exposure) const {
-    const isp_params& p = params.c;
-    FrameDesc f{};
-    f.width = raw.width;
-    f.height = raw.height;
-    f.raw = static_cast<const std::uint8_t*>(raw.data);
-    f.raw_pitch = static_cast<long long>(raw.pitch);
-    f.raw_format = raw.format;
-    f.cfa = 0;
-    for (int k = 0; k < 4; ++k) f.cfa |= kCfa[raw.pattern][k] << (2 * k);
-    const float white = p.white_level > 0.0F ? p.white_level
-                                             : static_cast<float>((1U << static_cast<unsigned>(raw.bit_depth)) - 1U);
-    for (int k = 0; k < 4; ++k) {
-      f.black[k] = p.has_channel_black_levels ? p.channel_black_levels[k] : p.black_level;
-      f.inv_range[k] = 1.0F / (white - f.black[k]);
-    }
-    f.white = static_cast<int>(std::ceil(white));
-    for (int k = 0; k < 3; ++k) f.wb[k] = p.wb_gains[k];
-    if (p.has_lens_shading) {
-      f.lsc = d_lsc;
-      f.lsc_width = p.lens_shading.width;
-      f.lsc_height = p.lens_shading.height;
-      f.lsc_channels = p.lens_shading.channels;
-      f.lsc_sx = static_cast<float>(p.lens_shading.width - 1) / static_cast<float>(raw.width - 1);
-      f.lsc_sy = static_cast<float>(p.lens_shading.height - 1) / static_cast<float>(raw.height - 1);
-    }
-    for (int k = 0; k < 9; ++k) f.ccm[k] = p.ccm[k];
-    f.lut = d_lut;
-    f.lut_size = static_cast<int>(lut.size());
-    f.sharpen = p.sharpen;
-    f.out = static_cast<std::uint8_t*>(out.data);
-    f.out_pitch = static_cast<long long>(out.pitch);
-    f.out_format = out.format;
-    f.histogram = stats != nullptr ? stats->histogram : nullptr;
-    f.stats = stats != nullptr ? stats->stats : nullptr;
-    f.bins = params.bins();
-    f.exposure_ns = exposure != nullptr ? exposure->exposure_ns : 0;
-    f.exposure_gain = exposure != nullptr ? exposure->gain : 0.0F;
-    f.ae = AeParams{p.auto_exposure.target,          p.auto_exposure.damping,  p.auto_exposure.min_exposure_ns,
-                    p.auto_exposure.max_exposure_ns, p.auto_exposure.min_gain, p.auto_exposure.max_gain};
-    f.wb_mode = p.white_balance;
-    f.percentile = p.white_patch_percentile;
-    f.smoothing = p.awb_smoothing;
-    return f;
-  }
-};
 
 extern "C" {
 
@@ -252,6 +98,39 @@ isp_status isp_process(isp_pipeline* pipeline, const isp_raw_view* raw, const is
     if (timing) pipeline->mark(s);
     check(hipEventRecord(pipeline->done, s), "hipEventRecord");
     pipeline->queued = true;
+  });
+}
+
+isp_status isp_set_exposure_feedback(isp_pipeline* pipeline, int32_t enable, const isp_frame_exposure* initial) {
+  return guarded([&] {
+    if (pipeline == nullptr) throw invalid_argument("pipeline is null");
+    if (pipeline->queued) check(hipEventSynchronize(pipeline->done), "hipEventSynchronize");
+    if (enable == 0) {
+      if (pipeline->d_ae != nullptr) check(hipFree(pipeline->d_ae), "hipFree");
+      pipeline->d_ae = nullptr;
+      return;
+    }
+    if (initial == nullptr || initial->exposure_ns <= 0) {
+      throw invalid_argument("exposure feedback needs an initial exposure_ns > 0");
+    }
+    if (!(initial->gain >= 0.0F) || !std::isfinite(initial->gain)) {
+      throw invalid_argument("exposure feedback needs a finite initial gain >= 0");
+    }
+    if (pipeline->d_ae == nullptr) check(hipMalloc(&pipeline->d_ae, sizeof(AeState)), "hipMalloc");
+    const AeState state{initial->exposure_ns, initial->gain, 1};
+    upload(&state, sizeof(state), pipeline->d_ae);
+  });
+}
+
+isp_status isp_get_exposure_state(isp_pipeline* pipeline, isp_frame_exposure* state) {
+  return guarded([&] {
+    if (pipeline == nullptr || state == nullptr) throw invalid_argument("pipeline or state is null");
+    if (pipeline->d_ae == nullptr) throw invalid_argument("exposure feedback is not enabled");
+    if (pipeline->queued) check(hipEventSynchronize(pipeline->done), "hipEventSynchronize");
+    AeState host{};
+    check(hipMemcpy(&host, pipeline->d_ae, sizeof(host), hipMemcpyDeviceToHost), "hipMemcpy");
+    state->exposure_ns = host.exposure_ns;
+    state->gain = host.gain;
   });
 }
 

@@ -227,3 +227,83 @@ class IspPipeline:
         e = _abi.FrameExposure(int(exposure[0]), float(exposure[1]))
         check(lib().isp_process(self._h, C.byref(raw_view), C.byref(rgb), C.byref(so), C.byref(e),
                                 stream), "process")
+
+    def set_exposure_feedback(self, initial):
+        """Device exposure feedback: each frame runs at its predecessor's suggestion,
+        the first at `initial` = (exposure_ns, gain); None switches it off."""
+        if initial is None:
+            check(lib().isp_set_exposure_feedback(self._h, 0, None), "set_exposure_feedback")
+        else:
+            e = _abi.FrameExposure(int(initial[0]), float(initial[1]))
+            check(lib().isp_set_exposure_feedback(self._h, 1, C.byref(e)), "set_exposure_feedback")
+
+    def exposure_state(self):
+        e = _abi.FrameExposure()
+        check(lib().isp_get_exposure_state(self._h, C.byref(e)), "exposure_state")
+        return int(e.exposure_ns), float(e.gain)
+
+
+def default_schedule():
+    s = _abi.BatchSchedule()
+    lib().isp_batch_schedule_init(C.byref(s))
+    return s
+
+
+class IspBatch:
+    """Many streams' frames in one persistent launch (isp_batch_*). Each frame is
+    (pipeline, raw_view, out, out_pitch, out_format, histogram, stats, exposure)."""
+
+    def __init__(self, schedule=None):
+        h = C.c_void_p()
+        check(lib().isp_batch_create(C.byref(h)), "IspBatch")
+        self._h = h
+        self._frames = None
+        if schedule is not None:
+            self.set_schedule(schedule)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().isp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def set_schedule(self, schedule):
+        check(lib().isp_batch_set_schedule(self._h, C.byref(schedule)), "batch set_schedule")
+
+    def schedule(self):
+        s = _abi.BatchSchedule()
+        check(lib().isp_batch_get_schedule(self._h, C.byref(s)), "batch schedule")
+        return s
+
+    def set_frames(self, frames):
+        arr = (_abi.BatchFrame * len(frames))()
+        for i, (pipe, raw_view, out, out_pitch, out_format, histogram, stats, exposure) in enumerate(frames):
+            arr[i].pipeline = pipe._h
+            arr[i].raw = raw_view
+            arr[i].out = _abi.RgbView(_ptr(out), out_pitch, out_format)
+            arr[i].stats = _abi.StatsOut(_ptr(histogram), _ptr(stats))
+            e = exposure if exposure is not None else (0, 0.0)
+            arr[i].exposure = _abi.FrameExposure(int(e[0]), float(e[1]))
+        check(lib().isp_batch_set_frames(self._h, arr, len(frames)), "batch set_frames")
+        self._frames = frames  # keeps the pipelines alive
+
+    def launch(self, stream=None):
+        check(lib().isp_batch_launch(self._h, stream), "batch launch")
+
+    def status(self):
+        """Waits for the launches; True if a launch gave up waiting for a white balance."""
+        flag = C.c_int32()
+        check(lib().isp_batch_status(self._h, C.byref(flag)), "batch status")
+        return bool(flag.value)
+
+    def tune(self, stream=None, cache_path=None, launches=5):
+        s = _abi.BatchSchedule()
+        ms = C.c_double()
+        path = cache_path.encode() if cache_path else None
+        check(lib().isp_batch_tune(self._h, stream, path, launches, C.byref(s), C.byref(ms)), "batch tune")
+        return s, ms.value

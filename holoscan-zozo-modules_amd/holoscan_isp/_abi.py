@@ -53,6 +53,10 @@ class Params(C.Structure):
                 ("histogram_bins", C.c_int32)]
 
 
+BATCH_MAX_FRAMES = 64
+BATCH_ORDER_LARGEST_FIRST, BATCH_ORDER_AS_GIVEN = 0, 1
+
+
 class RawView(C.Structure):
     _fields_ = [("data", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32),
                 ("pitch", C.c_size_t), ("format", C.c_uint8), ("bit_depth", C.c_int32),
@@ -71,9 +75,15 @@ class FrameExposure(C.Structure):
     _fields_ = [("exposure_ns", C.c_int64), ("gain", C.c_float)]
 
 
-class BatchItem(C.Structure):
-    _fields_ = [("stream_index", C.c_int32), ("raw", RawView), ("out", RgbView),
+class BatchFrame(C.Structure):
+    _fields_ = [("pipeline", C.c_void_p), ("raw", RawView), ("out", RgbView),
                 ("stats", StatsOut), ("exposure", FrameExposure)]
+
+
+class BatchSchedule(C.Structure):
+    _fields_ = [("tile_height", (C.c_int32 * 2) * 4), ("tiles_per_item", (C.c_int32 * 2) * 4),
+                ("awb_tiles_per_item", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("order", C.c_int32), ("graph", C.c_int32)]
 
 
 class SyntheticConfig(C.Structure):
@@ -128,6 +138,17 @@ PROTOTYPES = [
     ("isp_process", C.c_int, [_void, P(RawView), P(RgbView), P(StatsOut), P(FrameExposure), _void]),
     ("isp_timing_begin", C.c_int, [_void]),
     ("isp_timing_end", C.c_int, [_void, P(C.c_double), P(C.c_double), P(C.c_int32)]),
+    ("isp_set_exposure_feedback", C.c_int, [_void, C.c_int32, P(FrameExposure)]),
+    ("isp_get_exposure_state", C.c_int, [_void, P(FrameExposure)]),
+    ("isp_batch_schedule_init", None, [P(BatchSchedule)]),
+    ("isp_batch_create", C.c_int, [P(_void)]),
+    ("isp_batch_destroy", C.c_int, [_void]),
+    ("isp_batch_set_schedule", C.c_int, [_void, P(BatchSchedule)]),
+    ("isp_batch_get_schedule", C.c_int, [_void, P(BatchSchedule)]),
+    ("isp_batch_set_frames", C.c_int, [_void, P(BatchFrame), C.c_int32]),
+    ("isp_batch_launch", C.c_int, [_void, _void]),
+    ("isp_batch_status", C.c_int, [_void, P(C.c_int32)]),
+    ("isp_batch_tune", C.c_int, [_void, _void, C.c_char_p, C.c_int32, P(BatchSchedule), P(C.c_double)]),
     ("isp_synthetic_config_init", None, [P(SyntheticConfig)]),
     ("isp_synthetic_render", C.c_int, [P(SyntheticConfig), _void, C.c_size_t]),
     ("isp_synthetic_params", C.c_int, [P(SyntheticConfig), C.c_int32, P(Params), P(C.c_float)]),

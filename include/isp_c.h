@@ -230,6 +230,75 @@ isp_status isp_timing_begin(isp_pipeline* pipeline);
 isp_status isp_timing_end(isp_pipeline* pipeline, double* awb_ms, double* main_ms,
                           int32_t* frames);
 
+/* Device exposure feedback (no reference counterpart; the reference reports the
+ * auto-exposure suggestion and never applies it, isp_params.hpp:80-81). With
+ * `enable`, every frame of this pipeline runs at the exposure and gain its
+ * previous frame suggested (statistics entries 6 and 7, the exposure rounded to
+ * whole nanoseconds), the first at `initial`; the state stays on the device.
+ * Waits for the frames already queued. */
+isp_status isp_set_exposure_feedback(isp_pipeline* pipeline, int32_t enable, const isp_frame_exposure* initial);
+/* The exposure the pipeline's next frame will run at; waits for queued frames. */
+isp_status isp_get_exposure_state(isp_pipeline* pipeline, isp_frame_exposure* state);
+
+/* ---- the batched form (no reference counterpart) -------------------------
+ * Many camera streams in one persistent-kernel launch. Each frame of a batch
+ * belongs to a pipeline (its stream: parameters, tables, white balance and
+ * exposure state), at most one frame per pipeline, and its results are the
+ * ones isp_process gives for that pipeline. Frames of any raw format, size,
+ * pattern and parameters mix. The reference processes one frame per call
+ * (isp_pipeline.hpp:68-69). */
+#define ISP_BATCH_MAX_FRAMES 64
+
+typedef struct isp_batch isp_batch;
+
+typedef struct isp_batch_frame {
+  isp_pipeline* pipeline;
+  isp_raw_view raw;
+  isp_rgb_view out;
+  isp_stats_out stats; /* either pointer may be null */
+  isp_frame_exposure exposure;
+} isp_batch_frame;
+
+enum { ISP_BATCH_ORDER_LARGEST_FIRST = 0, ISP_BATCH_ORDER_AS_GIVEN = 1 };
+
+/* How a batch runs. The per-format schedule picks the main pass's tile height
+ * and work-item size for each raw format and demosaic; the rest applies to the
+ * whole launch. */
+typedef struct isp_batch_schedule {
+  int32_t tile_height[4][2];  /* [ISP_RAW_*][ISP_DEMOSAIC_*]: 16 or 32 output rows per tile */
+  int32_t tiles_per_item[4][2]; /* [ISP_RAW_*][ISP_DEMOSAIC_*]: main-pass tiles per work item, 1..32 */
+  int32_t awb_tiles_per_item; /* white-balance 64 x 64 tiles per work item, 1..64 */
+  int32_t blocks_per_cu;      /* resident blocks per CU; 0 = as many as fit */
+  int32_t order;              /* ISP_BATCH_ORDER_*: which frames' work the queue lists first */
+  int32_t graph;              /* 1: capture the launch as a hipGraph once and replay it */
+} isp_batch_schedule;
+
+/* The default schedule: 32-row tiles, 4 tiles per item, 8 white-balance tiles
+ * per item, full occupancy, largest frames first, graph replay. */
+void isp_batch_schedule_init(isp_batch_schedule* schedule);
+isp_status isp_batch_create(isp_batch** out);
+/* Waits for the batch's launches, then frees. */
+isp_status isp_batch_destroy(isp_batch* batch);
+isp_status isp_batch_set_schedule(isp_batch* batch, const isp_batch_schedule* schedule);
+isp_status isp_batch_get_schedule(const isp_batch* batch, isp_batch_schedule* schedule);
+/* Binds `count` frames (1..ISP_BATCH_MAX_FRAMES), validated as isp_process
+ * validates them. The binding holds until the next call; every launch processes
+ * the frames the buffers hold then. Waits for the batch's previous launch. */
+isp_status isp_batch_set_frames(isp_batch* batch, const isp_batch_frame* frames, int32_t count);
+/* Queues one launch of the bound frames on `stream`, after every member
+ * pipeline's queued frames; each pipeline's later frames wait for it. */
+isp_status isp_batch_launch(isp_batch* batch, isp_stream_t stream);
+/* Waits for the batch's launches; *wait_expired = 1 if a launch gave up waiting
+ * for a frame's white balance (its results are then not valid), and clears it. */
+isp_status isp_batch_status(isp_batch* batch, int32_t* wait_expired);
+/* Autotuner: times candidate schedules on the bound frames (`launches` timed
+ * launches each) and keeps the fastest; with `cache_path`, a schedule found for
+ * the same device and frame set is reused without timing, and new results are
+ * appended. The member pipelines' white balance and exposure state is restored
+ * afterwards. *best_ms: the chosen schedule's time per launch (0 from the cache). */
+isp_status isp_batch_tune(isp_batch* batch, isp_stream_t stream, const char* cache_path, int32_t launches,
+                          isp_batch_schedule* chosen, double* best_ms);
+
 #ifdef __cplusplus
 }
 #endif

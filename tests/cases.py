@@ -128,3 +128,42 @@ def config5(width=8192, height=6144, seed=5):
                       ccm=BASELINE_CCM, tone_curve=_abi.TONE_LUT, lut=srgb16_lut(),
                       histogram_bins=1024)
     return sc, p, _abi.OUT_RGB16
+
+
+def stream_ccm(i):
+    """A per-stream colour matrix with rows summing to 1: the benchmark CCM with stream-dependent
+    saturation."""
+    s = 1.0 + 0.05 * (i % 5)
+    base = BASELINE_CCM
+    rows = []
+    for r in range(3):
+        row = [base[3 * r + k] * s for k in range(3)]
+        row[r] += 1.0 - sum(row)
+        rows.extend(row)
+    return tuple(rows)
+
+
+def config3(divisor=1, seed=300):
+    """The 16-stream mixed batch: 4 x 1280x720 BGGR RAW10, 8 x 1920x1200 RGGB RAW12, 4 x 3840x2160
+    GBRG u16 at 14 bits; per-stream seed, black level, 17x13 lens shading and CCM; bilinear and
+    Malvar alternating by stream index; gray-world white balance; RGB8 + histograms + stats.
+    `divisor` scales the frame sizes down (even sizes kept) for checks against the oracle.
+    Returns [(scene, params, out_format)]."""
+    groups = [(4, 1280, 720, _abi.RAW_RAW10, 10, _abi.BGGR, 64.0, 4.0),
+              (8, 1920, 1200, _abi.RAW_RAW12, 12, _abi.RGGB, 240.0, 8.0),
+              (4, 3840, 2160, _abi.RAW_U16, 14, _abi.GBRG, 1024.0, 32.0)]
+    out = []
+    i = 0
+    for count, w, h, fmt, bits, pattern, black, step in groups:
+        for j in range(count):
+            width = max(2, (w // divisor) & ~1)
+            height = max(2, (h // divisor) & ~1)
+            bl = black + step * j
+            sc = baseline_scene(width, height, fmt, bits, pattern, bl, seed + i)
+            p = isp.IspParams(black_level=bl, lens_shading=isp.scene_lens_shading(sc, 17, 13),
+                              white_balance=_abi.WB_GRAY_WORLD,
+                              demosaic=_abi.BILINEAR if i % 2 == 0 else _abi.MALVAR,
+                              ccm=stream_ccm(i), tone_curve=_abi.TONE_SRGB)
+            out.append((sc, p, _abi.OUT_RGB8))
+            i += 1
+    return out

@@ -39,7 +39,8 @@ def test_library_loads_and_exports_every_declared_function():
 STRUCTS = {"isp_params": _abi.Params, "isp_lens_shading": _abi.LensShading,
            "isp_auto_exposure": _abi.AutoExposure, "isp_raw_view": _abi.RawView, "isp_rgb_view": _abi.RgbView,
            "isp_stats_out": _abi.StatsOut, "isp_frame_exposure": _abi.FrameExposure,
-           "isp_synthetic_config": _abi.SyntheticConfig, "isp_scene_config": _abi.SceneConfig}
+           "isp_synthetic_config": _abi.SyntheticConfig, "isp_scene_config": _abi.SceneConfig,
+           "isp_batch_frame": _abi.BatchFrame, "isp_batch_schedule": _abi.BatchSchedule}
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
@@ -59,3 +60,18 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         assert int(got[cname]) == C.sizeof(py), cname
         for field, _ in py._fields_:
             assert int(got[f"{cname}.{field}"]) == getattr(py, field).offset, (cname, field)
+
+
+def test_batch_schedule_defaults_without_a_device():
+    s = isp.default_schedule()
+    assert [list(r) for r in s.tile_height] == [[32, 32]] * 4
+    assert [list(r) for r in s.tiles_per_item] == [[4, 4]] * 4
+    assert (s.awb_tiles_per_item, s.blocks_per_cu, s.order, s.graph) == (8, 0, 0, 1)
+
+
+def test_batch_create_without_a_device_fails_cleanly():
+    if isp.lib().isp_probe_platform():
+        return
+    h = C.c_void_p()
+    assert isp.lib().isp_batch_create(C.byref(h)) == _abi.FAILURE
+    assert b"hip" in isp.lib().isp_last_error()
