@@ -33,6 +33,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-frames", type=int, default=6, help="frames of the single-thread CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--batch-ticks", type=int, default=100, help="timed ticks of the 16-stream batch (0: skip)")
+    ap.add_argument("--served-seconds", type=float, default=2.0,
+                    help="real-time length of the 32-camera serving trace (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times): the reference "
                          "point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -113,6 +116,160 @@ def cpu_baseline(frame, params, out_format, frames):
     dt2 = (time.perf_counter() - t0) / n
     out["all_cores"] = dict(value=pixels / dt2 / 1e9, unit="Gpix/s", cores=threads, kind="port",
                             ms_per_frame=dt2 * 1e3)
+    return out
+
+
+class StreamBuffers:
+    """One stream's frame resident in HBM and its output buffers."""
+
+    def __init__(self, isp, torch, scene, params, out_format):
+        from holoscan_isp.params import output_channel_bytes, output_channels
+        self.scene, self.params, self.out_format = scene, params, out_format
+        self.frame = isp.render_scene(scene)
+        self.d_raw = torch.from_numpy(self.frame.data).to("cuda")
+        self.view = self.frame.view(self.d_raw)
+        self.pitch = scene.width * output_channels(out_format) * output_channel_bytes(out_format)
+        self.out = torch.empty(self.pitch * scene.height, dtype=torch.uint8, device="cuda")
+        self.hist = torch.empty(4 * 256, dtype=torch.int32, device="cuda")
+        self.stats = torch.empty(12, dtype=torch.float32, device="cuda")
+
+
+def batched_vs_per_frame(ticks):
+    """The 16-stream mixed tick (4 x 720p RAW10, 8 x 1920x1200 RAW12, 4 x 4K u16 14-bit): 16
+    per-frame launches against one autotuned persistent launch replayed as a hipGraph. Per-tick
+    GPU time from HIP events on the launch stream."""
+    import torch
+    import holoscan_isp as isp
+    import cases
+    specs = cases.config3()
+    bufs = [StreamBuffers(isp, torch, *s) for s in specs]
+    pixels = sum(b.scene.width * b.scene.height for b in bufs)
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    exposure = (10_000_000, 2.0)
+
+    def timed(step):
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        evs = []
+        for _ in range(ticks):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            step()
+            e1.record(stream)
+            evs.append((e0, e1))
+        torch.cuda.synchronize()
+        ms = [a.elapsed_time(b) for a, b in evs]
+        mean = sum(ms) / len(ms)
+        return {"ms_per_tick": round(mean, 4), "gpix_s": round(pixels / mean / 1e6, 3),
+                "tick_ms_p50": round(percentile(ms, 50), 4), "tick_ms_p99": round(percentile(ms, 99), 4)}
+
+    pipes = [isp.IspPipeline(b.params) for b in bufs]
+
+    def per_frame():
+        for b, p in zip(bufs, pipes):
+            p.process(b.view, b.out, b.pitch, b.out_format, b.hist, b.stats, exposure, sh)
+
+    single = timed(per_frame)
+    bpipes = [isp.IspPipeline(b.params) for b in bufs]
+    batch = isp.IspBatch()
+    batch.set_frames([(p, b.view, b.out, b.pitch, b.out_format, b.hist, b.stats, exposure)
+                      for b, p in zip(bufs, bpipes)])
+    chosen, _ = batch.tune(sh, None, 5)
+    batched = timed(lambda: batch.launch(sh))
+    assert not batch.status()
+    batched["schedule"] = {"tile_height": [list(r) for r in chosen.tile_height],
+                           "tiles_per_item": [list(r) for r in chosen.tiles_per_item],
+                           "awb_tiles_per_item": chosen.awb_tiles_per_item, "blocks_per_cu": chosen.blocks_per_cu,
+                           "order": chosen.order, "graph": chosen.graph}
+    return {"workload": "16 streams: 4 x 1280x720 BGGR RAW10, 8 x 1920x1200 RGGB RAW12, 4 x 3840x2160 GBRG u16 "
+                        "14-bit; gray-world, bilinear/Malvar alternating, RGB8 + histograms + stats",
+            "pixels_per_tick": pixels, "ticks": ticks, "per_frame_launches": single, "batched": batched,
+            "speedup": round(single["ms_per_tick"] / batched["ms_per_tick"], 3)}
+
+
+def served(seconds, per_frame, device_source, saturate=False):
+    """32 cameras at 30 fps with Poisson arrivals (resolutions uniform over 720p / 1080p / 1920x1200
+    / 4K, RAW10 or RAW12, white patch 0.99, device exposure feedback), submitted in real time to
+    the serving loop; latency = submit to the loop seeing the launch finish. With `saturate`,
+    every frame of the trace is submitted at once, for the loop's capacity."""
+    import torch
+    import holoscan_isp as isp
+    from holoscan_isp import _abi
+    import cases
+    specs = cases.config4()
+    server = isp.IspServer(hbm_budget=4 << 30, max_batch=32, launch_slots=2, per_frame=1 if per_frame else 0)
+    ids = []
+    sources = []
+    for sc, p, of in specs:
+        ids.append(server.add_camera(p, sc.width, sc.height, sc.format, sc.bit_depth, sc.pattern, of, True,
+                                     (10_000_000, 2.0)))
+        f = isp.render_scene(sc)
+        if device_source:
+            buf = torch.from_numpy(f.data).to("cuda")
+        else:
+            buf = torch.empty(f.data.size, dtype=torch.uint8, pin_memory=True)
+            buf.numpy()[:] = f.data
+        sources.append((buf, f.pitch, sc.width * sc.height))
+    torch.cuda.synchronize()
+    location = _abi.LOCATION_DEVICE if device_source else _abi.LOCATION_HOST
+    events = cases.poisson_arrivals(len(specs), 30.0, seconds)
+    lat = []
+    pixels = 0
+    first = None
+    last = None
+    pending = 0
+
+    def drain(timeout_us=0):
+        nonlocal pending, pixels, last
+        while True:
+            r = server.poll(timeout_us)
+            if r is None:
+                return
+            if r.status != _abi.OK:
+                raise RuntimeError(isp.last_error())
+            lat.append((r.done_ns - r.submit_ns) / 1e6)
+            pixels += sources[r.camera][2]
+            last = r.done_ns if last is None else max(last, r.done_ns)
+            server.release(r.request)
+            pending -= 1
+            timeout_us = 0
+
+    t0 = isp.now_ns()
+    for t, c, k in events:
+        if not saturate:
+            due = t0 + int(t * 1e9)
+            while isp.now_ns() < due:
+                drain()
+        buf, pitch, _ = sources[c]
+        server.submit(ids[c], buf, pitch, location, tag=k)
+        first = isp.now_ns() if first is None else first
+        pending += 1
+        drain()
+    while pending > 0:
+        drain(1_000_000)
+    counters = server.counters()
+    server.close()
+    span = (last - first) / 1e9
+    return {"frames": len(lat), "latency_ms_p50": round(percentile(lat, 50), 4),
+            "latency_ms_p99": round(percentile(lat, 99), 4), "latency_ms_max": round(max(lat), 4),
+            "gpix_s": round(pixels / span / 1e9, 4), "launches": int(counters.launches),
+            "frames_per_launch": round(len(lat) / max(1, counters.launches), 2),
+            "max_launches_in_flight": int(counters.max_in_flight),
+            "budget_peak_mib": round(counters.budget_peak / 2 ** 20, 1)}
+
+
+def served_report(seconds):
+    out = {"workload": "32 cameras, 30 fps Poisson arrivals, 720p/1080p/1920x1200/4K RAW10/RAW12, white patch "
+                       "0.99, device exposure feedback, RGB8 + histograms + stats; 4 GiB budget, 2 launch slots, "
+                       "32 frames per launch", "seconds": seconds}
+    for source in ("device", "host_pinned"):
+        dev = source == "device"
+        out[source] = {"served": served(seconds, False, dev), "per_frame_launches": served(seconds, True, dev)}
+    cap_seconds = min(seconds, 1.0)
+    out["capacity_device"] = {"served": served(cap_seconds, False, True, saturate=True),
+                              "per_frame_launches": served(cap_seconds, True, True, saturate=True)}
     return out
 
 
@@ -223,6 +380,10 @@ def main():
                      "kernel": "main_kernel<malvar,sharpen>", "algorithmic_bytes_per_launch": algo_bytes,
                      "traffic_detail": traffic_detail},
     }
+    if args.batch_ticks > 0:
+        result["batched_16_streams"] = batched_vs_per_frame(args.batch_ticks)
+    if args.served_seconds > 0:
+        result["served_32_cameras"] = served_report(args.served_seconds)
     if not args.no_cpu:
         base = cpu_baseline(frame, params, out_format, args.cpu_frames)
         result["cpu_baseline"] = base["single"]

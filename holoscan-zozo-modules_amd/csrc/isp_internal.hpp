@@ -30,6 +30,9 @@ struct invalid_argument : std::invalid_argument {
 struct runtime_failure : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
+struct out_of_budget : std::runtime_error {  // ISP_OUT_OF_MEMORY
+  using std::runtime_error::runtime_error;
+};
 
 // Host-side helpers (isp_params.cpp).
 Params copy_params(const isp_params* params);  // validates

@@ -41,6 +41,9 @@ isp_status guarded(F&& body) {
   } catch (const invalid_argument& e) {
     set_error(e.what());
     return ISP_INVALID_ARGUMENT;
+  } catch (const out_of_budget& e) {
+    set_error(e.what());
+    return ISP_OUT_OF_MEMORY;
   } catch (const std::bad_alloc&) {
     set_error("IspPipeline (hip): out of host memory");
     return ISP_FAILURE;

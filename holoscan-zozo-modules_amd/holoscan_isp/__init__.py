@@ -307,3 +307,76 @@ class IspBatch:
         path = cache_path.encode() if cache_path else None
         check(lib().isp_batch_tune(self._h, stream, path, launches, C.byref(s), C.byref(ms)), "batch tune")
         return s, ms.value
+
+
+def server_config(**kw):
+    c = _abi.ServerConfig()
+    lib().isp_server_config_init(C.byref(c))
+    for key, value in kw.items():
+        setattr(c, key, value)
+    return c
+
+
+class IspServer:
+    """The served form (isp_server_*): cameras submit frames as they arrive; a native
+    scheduling thread admits them against an HBM budget and runs them in continuous batches."""
+
+    def __init__(self, config=None, **kw):
+        config = config if config is not None else server_config(**kw)
+        h = C.c_void_p()
+        check(lib().isp_server_create(C.byref(config), C.byref(h)), "IspServer")
+        self._h = h
+        self.config = config
+        self._cameras = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().isp_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def add_camera(self, params, width, height, fmt, bit_depth, pattern, out_format, stats=True,
+                   initial=(0, 0.0)):
+        c = _abi.CameraConfig()
+        cp, keep = params.to_c()
+        c.params = cp
+        c.width, c.height, c.format, c.bit_depth, c.pattern = width, height, fmt, bit_depth, pattern
+        c.out_format, c.stats = out_format, 1 if stats else 0
+        c.initial = _abi.FrameExposure(int(initial[0]), float(initial[1]))
+        cam = C.c_int32()
+        check(lib().isp_server_add_camera(self._h, C.byref(c), C.byref(cam)), "add_camera")
+        self._cameras.append((params, keep))
+        return cam.value
+
+    def submit(self, camera, raw, pitch, location=_abi.LOCATION_HOST, tag=0):
+        """`raw`: a host numpy array (pinned for an asynchronous copy) or a device tensor/address;
+        it must stay valid until the result is polled. Returns the request id."""
+        rid = C.c_int64()
+        check(lib().isp_server_submit(self._h, camera, _ptr(raw), pitch, location, tag, C.byref(rid)), "submit")
+        return rid.value
+
+    def poll(self, timeout_us=0):
+        """The next finished frame's ServerResult, or None."""
+        r = _abi.ServerResult()
+        status = lib().isp_server_poll(self._h, int(timeout_us), C.byref(r))
+        if status == _abi.NOT_READY:
+            return None
+        check(status, "poll")
+        return r
+
+    def release(self, request):
+        check(lib().isp_server_release(self._h, request), "release")
+
+    def counters(self):
+        c = _abi.ServerCounters()
+        check(lib().isp_server_counters_get(self._h, C.byref(c)), "counters")
+        return c
+
+
+def now_ns():
+    return lib().isp_server_now_ns()

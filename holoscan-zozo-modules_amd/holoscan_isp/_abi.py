@@ -86,6 +86,30 @@ class BatchSchedule(C.Structure):
                 ("order", C.c_int32), ("graph", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("launch_slots", C.c_int32),
+                ("per_frame", C.c_int32), ("schedule", BatchSchedule)]
+
+
+class CameraConfig(C.Structure):
+    _fields_ = [("params", Params), ("width", C.c_int32), ("height", C.c_int32), ("format", C.c_uint8),
+                ("bit_depth", C.c_int32), ("pattern", C.c_uint8), ("out_format", C.c_uint8),
+                ("stats", C.c_int32), ("initial", FrameExposure)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("request", C.c_int64), ("camera", C.c_int32), ("tag", C.c_uint64), ("status", C.c_int),
+                ("out", C.c_void_p), ("out_pitch", C.c_size_t), ("histogram", C.c_void_p),
+                ("stats", C.c_void_p), ("submit_ns", C.c_int64), ("launch_ns", C.c_int64),
+                ("done_ns", C.c_int64), ("batch_frames", C.c_int32)]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("launches", C.c_int64),
+                ("queued", C.c_int64), ("budget_used", C.c_uint64), ("budget_peak", C.c_uint64),
+                ("max_in_flight", C.c_int32)]
+
+
 class SyntheticConfig(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("pattern", C.c_uint8),
                 ("format", C.c_uint8), ("bit_depth", C.c_int32),
@@ -149,6 +173,15 @@ PROTOTYPES = [
     ("isp_batch_launch", C.c_int, [_void, _void]),
     ("isp_batch_status", C.c_int, [_void, P(C.c_int32)]),
     ("isp_batch_tune", C.c_int, [_void, _void, C.c_char_p, C.c_int32, P(BatchSchedule), P(C.c_double)]),
+    ("isp_server_config_init", None, [P(ServerConfig)]),
+    ("isp_server_create", C.c_int, [P(ServerConfig), P(_void)]),
+    ("isp_server_destroy", C.c_int, [_void]),
+    ("isp_server_add_camera", C.c_int, [_void, P(CameraConfig), P(C.c_int32)]),
+    ("isp_server_submit", C.c_int, [_void, C.c_int32, _void, C.c_size_t, C.c_uint8, C.c_uint64, P(C.c_int64)]),
+    ("isp_server_poll", C.c_int, [_void, C.c_int64, P(ServerResult)]),
+    ("isp_server_release", C.c_int, [_void, C.c_int64]),
+    ("isp_server_counters_get", C.c_int, [_void, P(ServerCounters)]),
+    ("isp_server_now_ns", C.c_int64, []),
     ("isp_synthetic_config_init", None, [P(SyntheticConfig)]),
     ("isp_synthetic_render", C.c_int, [P(SyntheticConfig), _void, C.c_size_t]),
     ("isp_synthetic_params", C.c_int, [P(SyntheticConfig), C.c_int32, P(Params), P(C.c_float)]),

@@ -299,6 +299,86 @@ isp_status isp_batch_status(isp_batch* batch, int32_t* wait_expired);
 isp_status isp_batch_tune(isp_batch* batch, isp_stream_t stream, const char* cache_path, int32_t launches,
                           isp_batch_schedule* chosen, double* best_ms);
 
+/* ---- the served form (no reference counterpart) ---------------------------
+ * A request loop on a thread of its own: frames from many cameras are submitted
+ * as they arrive and admitted first come, first served, as launch slots and HBM
+ * budget allow. Admission takes the device memory the frame needs (its raw copy
+ * and its outputs) from an arena of a fixed size. Whenever a launch slot is
+ * free, the admitted frames (at most one per camera and max_batch in all) run
+ * in one batched launch (isp_batch_*), overlapping the launches already on the
+ * GPU; requests arriving meanwhile join the next launch. Each camera is a
+ * pipeline: its white balance and exposure state stay on the device, and a
+ * camera's frames run in submission order. Results stay in the arena until
+ * released. The reference processes one frame per call (isp_op.cpp:559-566). */
+typedef struct isp_server isp_server;
+
+typedef struct isp_server_config {
+  uint64_t hbm_budget;         /* bytes of device memory for admitted frames */
+  int32_t max_batch;           /* frames per launch, 1..ISP_BATCH_MAX_FRAMES */
+  int32_t launch_slots;        /* launches in flight at once, each on a stream of its own, 1..8 */
+  int32_t per_frame;           /* 1: one per-frame launch (isp_process) per frame instead, as a baseline */
+  isp_batch_schedule schedule; /* of the batched launches */
+} isp_server_config;
+
+typedef struct isp_camera_config {
+  isp_params params;
+  int32_t width;
+  int32_t height;
+  uint8_t format;  /* ISP_RAW_* */
+  int32_t bit_depth;
+  uint8_t pattern; /* ISP_RGGB.. */
+  uint8_t out_format;
+  int32_t stats;              /* 1: histograms and the statistics vector per frame */
+  isp_frame_exposure initial; /* exposure_ns > 0: device exposure feedback starting here */
+} isp_camera_config;
+
+typedef struct isp_server_result {
+  int64_t request;
+  int32_t camera;
+  uint64_t tag;
+  isp_status status;
+  void* out; /* device buffers, valid until isp_server_release */
+  size_t out_pitch;
+  uint32_t* histogram;
+  float* stats;
+  int64_t submit_ns; /* isp_server_now_ns() clock */
+  int64_t launch_ns;
+  int64_t done_ns; /* when the loop saw the launch finish */
+  int32_t batch_frames;
+} isp_server_result;
+
+typedef struct isp_server_counters {
+  int64_t submitted;
+  int64_t completed;
+  int64_t launches;
+  int64_t queued;        /* waiting for admission now */
+  uint64_t budget_used;  /* bytes now */
+  uint64_t budget_peak;  /* the most bytes ever admitted at once */
+  int32_t max_in_flight; /* the most launches ever on the GPU at once */
+} isp_server_counters;
+
+/* 256 MiB budget, 32 frames per launch, 2 launch slots, batched, the default schedule. */
+void isp_server_config_init(isp_server_config* config);
+isp_status isp_server_create(const isp_server_config* config, isp_server** out);
+/* Finishes every submitted frame, then frees everything. */
+isp_status isp_server_destroy(isp_server* server);
+/* A camera whose frames are width x height in `format`; ISP_OUT_OF_MEMORY if one
+ * frame cannot fit the budget. */
+isp_status isp_server_add_camera(isp_server* server, const isp_camera_config* camera, int32_t* id);
+/* One frame of `camera`: its raw rows at `raw` (ISP_LOCATION_HOST: copied into
+ * the arena on admission, pinned memory for an asynchronous copy, and kept valid
+ * until the result is polled; ISP_LOCATION_DEVICE: read in place, valid until
+ * the result is polled). */
+isp_status isp_server_submit(isp_server* server, int32_t camera, const void* raw, size_t pitch, uint8_t location,
+                             uint64_t tag, int64_t* request);
+/* The next finished frame, waiting up to timeout_us; ISP_NOT_READY if none. */
+isp_status isp_server_poll(isp_server* server, int64_t timeout_us, isp_server_result* result);
+/* Returns a finished frame's arena memory. */
+isp_status isp_server_release(isp_server* server, int64_t request);
+isp_status isp_server_counters_get(isp_server* server, isp_server_counters* counters);
+/* The steady clock of the result timestamps, in nanoseconds. */
+int64_t isp_server_now_ns(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -167,3 +167,46 @@ def config3(divisor=1, seed=300):
             out.append((sc, p, _abi.OUT_RGB8))
             i += 1
     return out
+
+
+CONFIG4_SIZES = [(1280, 720), (1920, 1080), (1920, 1200), (3840, 2160)]
+
+
+def config4(cameras=32, seed=400):
+    """The served mix: `cameras` cameras at resolutions drawn uniformly from 720p, 1080p, 1920x1200
+    and 4K, RAW10 or RAW12 and a Bayer pattern per camera (seeded), white patch white balance at the
+    0.99 percentile, bilinear / Malvar alternating, 17x13 lens shading, per-camera CCM, sRGB, RGB8
+    with histograms and statistics. Returns [(scene, params, out_format)]."""
+    import random
+    rng = random.Random(seed)
+    out = []
+    for i in range(cameras):
+        w, h = CONFIG4_SIZES[rng.randrange(4)]
+        fmt = _abi.RAW_RAW10 if rng.randrange(2) == 0 else _abi.RAW_RAW12
+        bits = 10 if fmt == _abi.RAW_RAW10 else 12
+        pattern = rng.randrange(4)
+        black = 64.0 if bits == 10 else 240.0
+        sc = baseline_scene(w, h, fmt, bits, pattern, black, seed + i)
+        p = isp.IspParams(black_level=black, lens_shading=isp.scene_lens_shading(sc, 17, 13),
+                          white_balance=_abi.WB_WHITE_PATCH, white_patch_percentile=0.99,
+                          demosaic=_abi.BILINEAR if i % 2 == 0 else _abi.MALVAR,
+                          ccm=stream_ccm(i), tone_curve=_abi.TONE_SRGB)
+        out.append((sc, p, _abi.OUT_RGB8))
+    return out
+
+
+def poisson_arrivals(cameras, fps, duration, seed=401):
+    """Per camera, arrival times (s) of a Poisson process at `fps` over [0, duration); returns
+    [(t, camera, k)] sorted by time, k the camera's frame index."""
+    import random
+    rng = random.Random(seed)
+    events = []
+    for c in range(cameras):
+        t = rng.expovariate(fps)
+        k = 0
+        while t < duration:
+            events.append((t, c, k))
+            k += 1
+            t += rng.expovariate(fps)
+    events.sort()
+    return events

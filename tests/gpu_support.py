@@ -40,3 +40,23 @@ def mismatches(a, b):
 
 def same_stats(a, b):
     return np.array_equal(a.stats.view(np.uint32), b.stats.view(np.uint32))
+
+
+_hip = None
+
+
+def device_bytes(address, nbytes):
+    """Copies `nbytes` at a device address to a new host uint8 array (hipMemcpy of the HIP
+    runtime torch loaded)."""
+    import ctypes as C
+    import torch  # noqa: F401  (loads the HIP runtime first)
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMemcpy.restype = C.c_int
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    out = np.empty(nbytes, dtype=np.uint8)
+    if nbytes:
+        status = _hip.hipMemcpy(out.ctypes.data, address, nbytes, 2)  # hipMemcpyDeviceToHost
+        assert status == 0, f"hipMemcpy failed: {status}"
+    return out

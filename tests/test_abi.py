@@ -40,7 +40,9 @@ STRUCTS = {"isp_params": _abi.Params, "isp_lens_shading": _abi.LensShading,
            "isp_auto_exposure": _abi.AutoExposure, "isp_raw_view": _abi.RawView, "isp_rgb_view": _abi.RgbView,
            "isp_stats_out": _abi.StatsOut, "isp_frame_exposure": _abi.FrameExposure,
            "isp_synthetic_config": _abi.SyntheticConfig, "isp_scene_config": _abi.SceneConfig,
-           "isp_batch_frame": _abi.BatchFrame, "isp_batch_schedule": _abi.BatchSchedule}
+           "isp_batch_frame": _abi.BatchFrame, "isp_batch_schedule": _abi.BatchSchedule,
+           "isp_server_config": _abi.ServerConfig, "isp_camera_config": _abi.CameraConfig,
+           "isp_server_result": _abi.ServerResult, "isp_server_counters": _abi.ServerCounters}
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
@@ -75,3 +77,9 @@ def test_batch_create_without_a_device_fails_cleanly():
     h = C.c_void_p()
     assert isp.lib().isp_batch_create(C.byref(h)) == _abi.FAILURE
     assert b"hip" in isp.lib().isp_last_error()
+
+
+def test_server_config_defaults_without_a_device():
+    c = isp.server_config()
+    assert (c.hbm_budget, c.max_batch, c.launch_slots, c.per_frame) == (256 << 20, 32, 2, 0)
+    assert c.schedule.graph == 0 and c.schedule.tiles_per_item[3][1] == 4
