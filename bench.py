@@ -178,22 +178,23 @@ def batched_vs_per_frame(ticks):
                       for b, p in zip(bufs, bpipes)])
     chosen, _ = batch.tune(sh, None, 5)
     batched = timed(lambda: batch.launch(sh))
-    assert not batch.status()
+    batch.synchronize()
     batched["schedule"] = {"tile_height": [list(r) for r in chosen.tile_height],
                            "tiles_per_item": [list(r) for r in chosen.tiles_per_item],
                            "awb_tiles_per_item": chosen.awb_tiles_per_item, "blocks_per_cu": chosen.blocks_per_cu,
-                           "order": chosen.order, "graph": chosen.graph}
+                           "order": chosen.order, "queue": chosen.queue, "graph": chosen.graph}
     return {"workload": "16 streams: 4 x 1280x720 BGGR RAW10, 8 x 1920x1200 RGGB RAW12, 4 x 3840x2160 GBRG u16 "
                         "14-bit; gray-world, bilinear/Malvar alternating, RGB8 + histograms + stats",
             "pixels_per_tick": pixels, "ticks": ticks, "per_frame_launches": single, "batched": batched,
             "speedup": round(single["ms_per_tick"] / batched["ms_per_tick"], 3)}
 
 
-def served(seconds, per_frame, device_source, saturate=False):
+def served(seconds, per_frame, device_source, saturate=False, fps=30.0):
     """32 cameras at 30 fps with Poisson arrivals (resolutions uniform over 720p / 1080p / 1920x1200
     / 4K, RAW10 or RAW12, white patch 0.99, device exposure feedback), submitted in real time to
     the serving loop; latency = submit to the loop seeing the launch finish. With `saturate`,
-    every frame of the trace is submitted at once, for the loop's capacity."""
+    every frame of the trace is submitted back to back before any result is collected, for
+    the loop's capacity."""
     import torch
     import holoscan_isp as isp
     from holoscan_isp import _abi
@@ -214,7 +215,7 @@ def served(seconds, per_frame, device_source, saturate=False):
         sources.append((buf, f.pitch, sc.width * sc.height))
     torch.cuda.synchronize()
     location = _abi.LOCATION_DEVICE if device_source else _abi.LOCATION_HOST
-    events = cases.poisson_arrivals(len(specs), 30.0, seconds)
+    events = cases.poisson_arrivals(len(specs), fps, seconds)
     lat = []
     pixels = 0
     first = None
@@ -246,7 +247,8 @@ def served(seconds, per_frame, device_source, saturate=False):
         server.submit(ids[c], buf, pitch, location, tag=k)
         first = isp.now_ns() if first is None else first
         pending += 1
-        drain()
+        if not saturate:
+            drain()
     while pending > 0:
         drain(1_000_000)
     counters = server.counters()
@@ -268,6 +270,11 @@ def served_report(seconds):
         dev = source == "device"
         out[source] = {"served": served(seconds, False, dev), "per_frame_launches": served(seconds, True, dev)}
     cap_seconds = min(seconds, 1.0)
+    for scale in (4, 10):
+        sec = min(seconds, 0.5)
+        out[f"device_{scale}x_rate"] = {"fps": 30 * scale, "seconds": sec,
+                                       "served": served(sec, False, True, fps=30.0 * scale),
+                                       "per_frame_launches": served(sec, True, True, fps=30.0 * scale)}
     out["capacity_device"] = {"served": served(cap_seconds, False, True, saturate=True),
                               "per_frame_launches": served(cap_seconds, True, True, saturate=True)}
     return out

@@ -1,7 +1,8 @@
 // isp_batch.hip - the batched form (include/isp_c.h, isp_batch_*): many camera
-// streams, each a pipeline with its own tables and state, in one persistent
-// launch of batch_kernel (isp_kernels.hip), optionally replayed as a hipGraph,
-// with an autotuner over the launch schedule.
+// streams, each a pipeline with its own tables and state, in two persistent
+// launches (white balance statistics of every auto-balanced frame, then every
+// frame's main pass; isp_kernels.hip), optionally replayed as a hipGraph, with
+// an autotuner over the launch schedule.
 //
 // The host builds the work queue's table (one BatchFrame per frame plus the
 // order in which the queue lists them) in pinned memory and copies it to the
@@ -31,6 +32,7 @@ void schedule_defaults(isp_batch_schedule* s) {
   s->awb_tiles_per_item = kDefaultAwbTilesPerItem;
   s->blocks_per_cu = 0;
   s->order = ISP_BATCH_ORDER_LARGEST_FIRST;
+  s->queue = ISP_BATCH_QUEUE_DYNAMIC;
   s->graph = 1;
 }
 
@@ -52,6 +54,9 @@ void validate_schedule(const isp_batch_schedule& s) {
   if (s.order != ISP_BATCH_ORDER_LARGEST_FIRST && s.order != ISP_BATCH_ORDER_AS_GIVEN) {
     throw invalid_argument("schedule.order is not an ISP_BATCH_ORDER");
   }
+  if (s.queue < ISP_BATCH_QUEUE_DYNAMIC || s.queue > ISP_BATCH_QUEUE_STATIC_XCD) {
+    throw invalid_argument("schedule.queue is not an ISP_BATCH_QUEUE");
+  }
   if (s.graph != 0 && s.graph != 1) throw invalid_argument("schedule.graph must be 0 or 1");
 }
 
@@ -72,11 +77,12 @@ struct isp_batch {
   BatchTable* d_table = nullptr;
   BatchControl* d_control = nullptr;
   std::size_t table_bytes = 0;
-  bool dirty = false;  // the host table is newer than the device's
+  bool dirty = false;    // the host table is newer than the device's
+  bool sharpen = false;  // a frame sharpens: the batch kernel with the larger LDS
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_grid = 0;
+  int graph_grid = 0;  // the grids the graph was captured with
   hipEvent_t done = nullptr;
   bool queued = false;
 
@@ -110,12 +116,13 @@ struct isp_batch {
     if (queued) check(hipEventSynchronize(done), "hipEventSynchronize");
   }
 
-  int grid() const {
-    const int fit = batch_blocks_per_cu();
-    const int per_cu = schedule.blocks_per_cu > 0 ? std::min(schedule.blocks_per_cu, fit) : fit;
-    int items = h_table->total_items;
-    return std::max(1, std::min(items, cu_count * per_cu));
+  int resident_per_cu() const {
+    const int fit = batch_main_blocks_per_cu(sharpen);
+    return schedule.blocks_per_cu > 0 ? std::min(schedule.blocks_per_cu, fit) : fit;
   }
+
+  int main_grid() const { return std::max(1, std::min(h_table->main_items, cu_count * resident_per_cu())); }
+  int awb_grid() const { return std::max(1, std::min(h_table->awb_items, cu_count * batch_awb_blocks_per_cu())); }
 
   // Builds the host table for `frames` under `schedule`.
   void build() {
@@ -123,6 +130,9 @@ struct isp_batch {
     const int n = static_cast<int>(frames.size());
     std::memset(&t, 0, sizeof(BatchTable));
     t.count = n;
+    t.queue = schedule.queue;
+    sharpen = false;
+    for (const isp_batch_frame& fr : frames) sharpen = sharpen || fr.pipeline->params.c.sharpen > 0.0F;
     std::vector<double> cost(n);
     for (int i = 0; i < n; ++i) {
       const isp_batch_frame& fr = frames[i];
@@ -149,6 +159,28 @@ struct isp_batch {
       b.out_aligned = aligned16(fr.out.data, static_cast<long long>(fr.out.pitch)) ? 1 : 0;
       cost[i] = static_cast<double>(fr.raw.width) * fr.raw.height * (sharpen ? 1.6 : 1.0) * (malvar ? 1.2 : 1.0);
     }
+    // A small launch gets smaller work items, so that every resident block has
+    // work: the items per pass are kept at twice the blocks or more when the
+    // tiles allow it, never above the schedule's item size.
+    const int capacity = cu_count * resident_per_cu();
+    const int awb_capacity = cu_count * batch_awb_blocks_per_cu();
+    long long main_tiles = 0;
+    long long awb_tiles = 0;
+    for (int i = 0; i < n; ++i) {
+      main_tiles += t.frames[i].tiles;
+      awb_tiles += t.frames[i].awb_tiles;
+    }
+    const int main_cap = static_cast<int>(std::max(1LL, main_tiles / (2LL * capacity)));
+    const int awb_cap = static_cast<int>(std::max(1LL, awb_tiles / (2LL * awb_capacity)));
+    for (int i = 0; i < n; ++i) {
+      BatchFrame& b = t.frames[i];
+      b.tiles_per_item = std::min(b.tiles_per_item, main_cap);
+      b.items = ceil_div(b.tiles, b.tiles_per_item);
+      if (b.awb_tiles > 0) {
+        b.awb_tiles_per_item = std::min(b.awb_tiles_per_item, awb_cap);
+        b.awb_items = ceil_div(b.awb_tiles, b.awb_tiles_per_item);
+      }
+    }
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     if (schedule.order == ISP_BATCH_ORDER_LARGEST_FIRST) {
@@ -165,7 +197,7 @@ struct isp_batch {
       main += t.frames[order[k]].items;
     }
     t.awb_items = awb;
-    t.total_items = awb + main;
+    t.main_items = main;
     table_bytes = offsetof(BatchTable, frames) + static_cast<std::size_t>(n) * sizeof(BatchFrame);
     dirty = true;
     drop_graph();  // the grid may change with the items
@@ -182,13 +214,18 @@ struct isp_batch {
       check(hipMemcpyAsync(d_table, h_table, table_bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
       dirty = false;
     }
-    const int blocks = grid();
+    const int awb_blocks = h_table->awb_items > 0 ? awb_grid() : 0;
+    const int main_blocks = main_grid();
+    const auto enqueue = [&](hipStream_t st) {
+      if (awb_blocks > 0) launch_batch_awb(d_table, d_control, awb_blocks, st);
+      launch_batch_main(d_table, d_control, main_blocks, sharpen, st);
+    };
     if (schedule.graph != 0) {
-      if (exec == nullptr || graph_grid != blocks) {
+      if (exec == nullptr || graph_grid != main_blocks * 65536 + awb_blocks) {
         drop_graph();
         check(hipStreamBeginCapture(capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
-          launch_batch(d_table, d_control, blocks, capture);
+          enqueue(capture);
         } catch (...) {
           hipGraph_t partial = nullptr;
           static_cast<void>(hipStreamEndCapture(capture, &partial));
@@ -197,11 +234,11 @@ struct isp_batch {
         }
         check(hipStreamEndCapture(capture, &graph), "hipStreamEndCapture");
         check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        graph_grid = blocks;
+        graph_grid = main_blocks * 65536 + awb_blocks;
       }
       check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
     } else {
-      launch_batch(d_table, d_control, blocks, stream);
+      enqueue(stream);
     }
     check(hipEventRecord(done, stream), "hipEventRecord");
     queued = true;
@@ -238,7 +275,7 @@ std::string schedule_text(const isp_batch_schedule& s) {
   std::ostringstream o;
   for (const auto& row : s.tile_height) o << row[0] << ' ' << row[1] << ' ';
   for (const auto& row : s.tiles_per_item) o << row[0] << ' ' << row[1] << ' ';
-  o << s.awb_tiles_per_item << ' ' << s.blocks_per_cu << ' ' << s.order;
+  o << s.awb_tiles_per_item << ' ' << s.blocks_per_cu << ' ' << s.order << ' ' << s.queue;
   return o.str();
 }
 
@@ -247,7 +284,7 @@ bool parse_schedule(const std::string& text, isp_batch_schedule* s) {
   isp_batch_schedule r{};
   for (auto& row : r.tile_height) in >> row[0] >> row[1];
   for (auto& row : r.tiles_per_item) in >> row[0] >> row[1];
-  in >> r.awb_tiles_per_item >> r.blocks_per_cu >> r.order;
+  in >> r.awb_tiles_per_item >> r.blocks_per_cu >> r.order >> r.queue;
   if (!in) return false;
   r.graph = s->graph;
   try {
@@ -363,14 +400,10 @@ isp_status isp_batch_launch(isp_batch* batch, isp_stream_t stream) {
   });
 }
 
-isp_status isp_batch_status(isp_batch* batch, int32_t* wait_expired) {
+isp_status isp_batch_synchronize(isp_batch* batch) {
   return guarded([&] {
     if (batch == nullptr) throw invalid_argument("batch is null");
     batch->wait();
-    unsigned int error = 0;
-    check(hipMemcpy(&error, &batch->d_control->error, sizeof(error), hipMemcpyDeviceToHost), "hipMemcpy");
-    if (error != 0) check(hipMemset(&batch->d_control->error, 0, sizeof(error)), "hipMemset");
-    if (wait_expired != nullptr) *wait_expired = error != 0 ? 1 : 0;
   });
 }
 
@@ -468,10 +501,20 @@ isp_status isp_batch_tune(isp_batch* batch, isp_stream_t stream, const char* cac
         c.order = best.order == ISP_BATCH_ORDER_LARGEST_FIRST ? ISP_BATCH_ORDER_AS_GIVEN : ISP_BATCH_ORDER_LARGEST_FIRST;
         consider(c);
       }
-      if (batch_blocks_per_cu() > 1) {
+      for (int qm : {ISP_BATCH_QUEUE_DYNAMIC, ISP_BATCH_QUEUE_STATIC, ISP_BATCH_QUEUE_STATIC_XCD}) {
+        if (qm == best.queue) continue;
         isp_batch_schedule c = best;
-        c.blocks_per_cu = best.blocks_per_cu == 1 ? 0 : 1;
+        c.queue = qm;
         consider(c);
+      }
+      {
+        const int fit = batch_main_blocks_per_cu(batch->sharpen);
+        for (int bp : {1, 2}) {
+          if (bp >= fit || bp == best.blocks_per_cu) continue;
+          isp_batch_schedule c = best;
+          c.blocks_per_cu = bp;
+          consider(c);
+        }
       }
     } catch (...) {
       static_cast<void>(hipEventDestroy(t0));

@@ -143,14 +143,14 @@ struct BatchFrame {
   int out_aligned;
 };
 
-// The work queue's layout: all white-balance items first (frames in awb_order),
-// then all main-pass items (frames in main_order). start[k] is the first item
-// of the k-th frame in that order, relative to the start of its pass.
+// The work queues of a batch's two passes: the white balance pass lists the
+// auto-balanced frames' items in awb_order, the main pass every frame's items in
+// main_order. start[k] is the first item of the k-th frame in that order.
 struct BatchTable {
   int count;
   int awb_items;
-  int total_items;
-  int pad;
+  int main_items;
+  int queue;  // ISP_BATCH_QUEUE_*
   int awb_start[kMaxBatch];
   int awb_order[kMaxBatch];
   int main_start[kMaxBatch];
@@ -159,13 +159,12 @@ struct BatchTable {
 };
 
 // Device control block of a batch, zero at creation; every launch leaves it
-// ready for the next (the last block to exit resets the queue).
+// ready for the next (the last block of a pass resets its queue).
 struct BatchControl {
-  unsigned int next;              // work queue head
-  unsigned int exited;            // blocks that left the queue
-  unsigned int tick;              // launches completed
-  unsigned int error;             // set when a bounded wait gave up
-  unsigned int ready[kMaxBatch];  // tick + 1 once frame i's white balance is final
+  unsigned int awb_next;
+  unsigned int awb_exited;
+  unsigned int main_next;
+  unsigned int main_exited;
 };
 
 // Kernel launchers (isp_kernels.hip). `stream` is a hipStream_t.
@@ -177,10 +176,12 @@ struct LaunchConfig {
 void launch_awb(const FrameDesc& f, DeviceScratch* scratch, const LaunchConfig& cfg, void* stream);
 void launch_main(const FrameDesc& f, DeviceScratch* scratch, bool malvar, bool sharpen,
                  const LaunchConfig& cfg, void* stream);
-// The batch kernel's resident blocks per CU, and its launch (grid blocks).
-int batch_blocks_per_cu();
-void launch_batch(const BatchTable* table, BatchControl* control, int grid, void* stream);
-// Bounded wait of the batch kernel, in sleep rounds (tests shorten it).
-void set_batch_spin_limit(unsigned int rounds);
+// The batch kernels' resident blocks per CU and their launches (grid blocks);
+// the main-pass kernel for batches with sharpened frames differs from the one
+// without.
+int batch_awb_blocks_per_cu();
+int batch_main_blocks_per_cu(bool sharpen);
+void launch_batch_awb(const BatchTable* table, BatchControl* control, int grid, void* stream);
+void launch_batch_main(const BatchTable* table, BatchControl* control, int grid, bool sharpen, void* stream);
 
 }  // namespace isp_amd

@@ -32,17 +32,16 @@
 //               (finish_stats, isp_math.hpp:387-425, with the exact log2/exp2
 //               of isp_ae.hpp).
 //
-// Batched form, one launch for many frames (batch_kernel): a persistent grid
-// takes work items (a few tiles of one frame) from a device queue that lists
-// every frame's white-balance items first and then every frame's main-pass
-// items. The same tile bodies run per item, chosen per frame among all raw
-// formats, both demosaics, with and without sharpening and two tile heights. A
-// block keeps one frame's partial sums and histograms in LDS while it takes
-// consecutive items of that frame, and adds them to the frame's scratch when it
-// moves on; the block whose contribution completes a frame's item count runs
-// that frame's epilogue. A main-pass item of an auto-white-balanced frame waits
-// (bounded) for that frame's gains: every white-balance item was taken before
-// any main-pass item, by a block that never waits, so the wait always ends.
+// Batched form, many frames in two persistent launches (batch_awb_kernel, then
+// batch_main_kernel): each pass's grid takes work items (a few tiles of one
+// frame) from a device queue that lists the frames largest first. The same
+// tile bodies run per item, chosen per frame among all raw formats, both
+// demosaics, with and without sharpening and two tile heights. A block keeps
+// one frame's partial sums and histograms in LDS while it takes consecutive
+// items of that frame and adds them to the frame's scratch when it moves on;
+// the block whose contribution completes a frame's item count runs that
+// frame's epilogue. The kernel boundary between the passes orders each frame's
+// white balance before its main pass.
 //
 // Lens shading: the reference's per-sample bilinear interpolation
 // (lens_shading_gain, isp_math.hpp:121-140) is top + (bottom - top) * ty with
@@ -57,7 +56,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <climits>
 #include <map>
 #include <mutex>
@@ -173,11 +171,8 @@ __device__ __forceinline__ unsigned int fold(unsigned long long v) {
   return static_cast<unsigned int>(v) ^ static_cast<unsigned int>(v >> 32);
 }
 
-// A float read and written at the point of coherence (state shared by blocks of
-// one launch).
-__device__ __forceinline__ float coherent_read(const float* p) {
-  return __uint_as_float(atomicOr(reinterpret_cast<unsigned int*>(const_cast<float*>(p)), 0U));
-}
+// The white balance state is written with device-scope atomics (performed at
+// the point of coherence), as the statistics scratch is.
 __device__ __forceinline__ unsigned int coherent_write(float* p, float v) {
   return atomicExch(reinterpret_cast<unsigned int*>(p), __float_as_uint(v));
 }
@@ -538,10 +533,8 @@ __device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl
 }
 
 // The frame's last contributor: finish_white_balance (isp_math.hpp:357-384)
-// into the stream's white balance state; the reads clear the scratch. With
-// `ready`, the final gains are published there as `ready_value` once written.
-__device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s, AwbFrameLds& fl, const AwbCtx& c,
-                                           unsigned int* ready, unsigned int ready_value) {
+// into the stream's white balance state; the reads clear the scratch.
+__device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s, AwbFrameLds& fl, const AwbCtx& c) {
   const int tid = threadIdx.x;
   if (tid < 4) fl.sums[tid] = atomicExch(tid < 3 ? &s->awb.sum[tid] : &s->awb.quads, 0ULL);
   if (c.patch) {
@@ -584,10 +577,7 @@ __device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s,
         token ^= atomicExch(reinterpret_cast<unsigned int*>(&st.valid), 1U);
       }
     }
-    if (ready != nullptr) {
-      __asm__ volatile("" : : "v"(token));  // the gains are written before they are announced
-      atomicExch(ready, ready_value);
-    }
+    __asm__ volatile("" : : "v"(token));  // the state is written before the ticket clears
     atomicExch(&s->awb.done, 0U);
   }
 }
@@ -603,7 +593,7 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) awb_tile<kFmt>(f, c, fl, tl, tile, tiles_x, fast_rows, local);
   const unsigned int token = awb_flush(&s->awb, fl, c, local);
   if (!complete(&s->awb.done, 1U, gridDim.x, token, &fl.flag)) return;
-  awb_finish(f, s, fl, c, nullptr, 0U);
+  awb_finish(f, s, fl, c);
 }
 
 // ---- the main pass ----------------------------------------------------------
@@ -1283,13 +1273,22 @@ constexpr std::size_t max_size() {
   }
 }
 
-// The largest tile LDS of every body the batch kernel runs (sharpening with
-// 32-row tiles dominates; RAW10's 16-sample chunks widen its lens shading table).
-constexpr std::size_t kBatchTileLds =
-    max_size<AwbTileLds, MainTileLds<ISP_RAW_U8, true, true, 32>, MainTileLds<ISP_RAW_U16, true, true, 32>,
-             MainTileLds<ISP_RAW_RAW10, true, true, 32>, MainTileLds<ISP_RAW_RAW12, true, true, 32>,
-             MainTileLds<ISP_RAW_RAW10, false, true, 32>, MainTileLds<ISP_RAW_RAW12, false, true, 32>>();
-constexpr std::size_t kBatchFrameLds = max_size<AwbFrameLds, MainFrameLds>();
+// The largest tile LDS of every body a main-pass batch kernel runs: with
+// sharpening, 32-row tiles dominate (RAW10's 16-sample chunks widen its lens
+// shading table); without it the LDS is small enough for 3 blocks per CU.
+template <bool kSharpen>
+constexpr std::size_t batch_tile_lds() {
+  if constexpr (kSharpen) {
+    return max_size<MainTileLds<ISP_RAW_U8, true, true, 32>, MainTileLds<ISP_RAW_U16, true, true, 32>,
+                    MainTileLds<ISP_RAW_RAW10, true, true, 32>, MainTileLds<ISP_RAW_RAW12, true, true, 32>,
+                    MainTileLds<ISP_RAW_RAW10, false, true, 32>, MainTileLds<ISP_RAW_RAW12, false, true, 32>,
+                    MainTileLds<ISP_RAW_RAW10, true, false, 32>>();
+  } else {
+    return max_size<MainTileLds<ISP_RAW_U8, true, false, 32>, MainTileLds<ISP_RAW_U16, true, false, 32>,
+                    MainTileLds<ISP_RAW_RAW10, true, false, 32>, MainTileLds<ISP_RAW_RAW12, true, false, 32>,
+                    MainTileLds<ISP_RAW_RAW10, false, false, 32>, MainTileLds<ISP_RAW_RAW12, false, false, 32>>();
+  }
+}
 
 constexpr int variant_of(int fmt, bool malvar, bool sharpen, int tile_h) {
   return fmt | (malvar ? 4 : 0) | (sharpen ? 8 : 0) | (tile_h == 16 ? 16 : 0);
@@ -1308,172 +1307,231 @@ __device__ __forceinline__ void batch_main_item(const BatchFrame& b, const MainC
 
 template <int kFmt>
 __device__ __forceinline__ void batch_awb_item(const BatchFrame& b, const AwbCtx& c, AwbFrameLds& fl,
-                                               unsigned char* tile_lds, int item, unsigned long long acc[4]) {
-  auto& tl = *reinterpret_cast<AwbTileLds*>(tile_lds);
+                                               AwbTileLds& tl, int item, unsigned long long acc[4]) {
   const int first = item * b.awb_tiles_per_item;
   const int end = min(first + b.awb_tiles_per_item, b.awb_tiles);
   for (int t = first; t < end; ++t) awb_tile<kFmt>(b.f, c, fl, tl, t, b.awb_tiles_x, b.fast_rows, acc);
 }
 
-#define ISP_BATCH_MAIN_CASE(FMT, MAL, SHP, TH)                                             \
-  case variant_of(FMT, MAL, SHP, TH):                                                      \
-    batch_main_item<FMT, MAL, SHP, TH>(b, mc, mfl, tile_lds, local, acc);                  \
-    break;
-#define ISP_BATCH_MAIN_CASES(FMT)                                                         \
-  ISP_BATCH_MAIN_CASE(FMT, false, false, 32)                                              \
-  ISP_BATCH_MAIN_CASE(FMT, false, true, 32)                                               \
-  ISP_BATCH_MAIN_CASE(FMT, true, false, 32)                                               \
-  ISP_BATCH_MAIN_CASE(FMT, true, true, 32)                                                \
-  ISP_BATCH_MAIN_CASE(FMT, false, false, 16)                                              \
-  ISP_BATCH_MAIN_CASE(FMT, false, true, 16)                                               \
-  ISP_BATCH_MAIN_CASE(FMT, true, false, 16)                                               \
-  ISP_BATCH_MAIN_CASE(FMT, true, true, 16)
-
-// The frame (in the pass's order) that holds queue item `local` of a pass, by one
+// The frame (in the pass's order) that holds item `item` of a pass, by one
 // coalesced load of the pass's start table per wave; returns its index in the
 // table and the item within the frame.
-__device__ __forceinline__ int batch_locate(const int* start, const int* order, int count, int local, int* within) {
+__device__ __forceinline__ int batch_locate(const int* start, const int* order, int count, int item, int* within) {
   const int lane = threadIdx.x & 63;
   const int s = lane < count ? start[lane] : INT_MAX;
-  const unsigned long long mask = __ballot(s <= local);
+  const unsigned long long mask = __ballot(s <= item);
   const int k = 63 - __clzll(static_cast<long long>(mask));
   const int ks = __builtin_amdgcn_readfirstlane(k);
-  *within = local - start[ks];
+  *within = item - start[ks];
   return order[ks];
 }
 
-__global__ void __launch_bounds__(kThreads) batch_kernel(const BatchTable* __restrict__ table, BatchControl* ctl,
-                                                         unsigned int spin_limit) {
-  __shared__ __attribute__((aligned(16))) unsigned char frame_lds[kBatchFrameLds];
-  __shared__ __attribute__((aligned(16))) unsigned char tile_lds[kBatchTileLds];
-  __shared__ int slot[2];  // the next work item; the launch's tick
-  auto& afl = *reinterpret_cast<AwbFrameLds*>(frame_lds);
-  auto& mfl = *reinterpret_cast<MainFrameLds*>(frame_lds);
-  const int tid = threadIdx.x;
-  const int count = table->count;
-  const int awb_items = table->awb_items;
-  const int total = table->total_items;
+// The work of one pass over a batch, by the table's queue mode:
+//  dynamic     every block's first item is its own (blockIdx.x); later ones come
+//              from an atomic counter, fetched while the current item runs, and
+//              the last block to leave resets the counter for the next launch
+//              (so a captured launch replays without a memset);
+//  static      block b takes items b, b + grid, ... (no atomics);
+//  static XCD  the items split into one contiguous range per XCD (blocks are
+//              dispatched to the 8 XCDs in turn, block b to XCD b % 8), each
+//              range taken in turn by that XCD's blocks, so neighbouring tiles,
+//              whose halos overlap, meet in the same L2.
+constexpr int kXcds = 8;
+
+struct BatchQueue {
+  unsigned int* next;
+  unsigned int* exited;
+  int total;
+  int mode;   // ISP_BATCH_QUEUE_*
+  int* slot;  // block-shared
   unsigned int pending = 0;
-  if (tid == 0) {
-    slot[1] = static_cast<int>(atomicOr(&ctl->tick, 0U));
-    slot[0] = static_cast<int>(atomicAdd(&ctl->next, 1U));
+  int step = 0;  // static XCD: items between this block's turns, and its range's end
+  int end = 0;
+
+  __device__ int first() {
+    if (mode == ISP_BATCH_QUEUE_STATIC) return static_cast<int>(blockIdx.x);
+    if (mode == ISP_BATCH_QUEUE_STATIC_XCD) {
+      const int g = static_cast<int>(gridDim.x);
+      const int parts = g < kXcds ? g : kXcds;
+      const int x = static_cast<int>(blockIdx.x) % parts;
+      const int j = static_cast<int>(blockIdx.x) / parts;
+      step = (g - x + parts - 1) / parts;
+      const int begin = static_cast<int>(static_cast<long long>(total) * x / parts);
+      end = static_cast<int>(static_cast<long long>(total) * (x + 1) / parts);
+      return begin + j < end ? begin + j : total;
+    }
+    if (threadIdx.x == 0) *slot = static_cast<int>(blockIdx.x);
+    __syncthreads();
+    const int item = *slot;
+    __syncthreads();
+    return item;
   }
-  int cur = -1;    // frame of the open contribution
-  int cur_pass = 0;  // its pass: 0 white balance, 1 main
+  // Dynamic: fetches the item after the current one (thread 0; read by next_item()).
+  __device__ void prefetch() {
+    if (mode == ISP_BATCH_QUEUE_DYNAMIC && threadIdx.x == 0) pending = gridDim.x + atomicAdd(next, 1U);
+  }
+  __device__ int next_item(int item) {
+    if (mode == ISP_BATCH_QUEUE_STATIC) return item + static_cast<int>(gridDim.x);
+    if (mode == ISP_BATCH_QUEUE_STATIC_XCD) return item + step < end ? item + step : total;
+    if (threadIdx.x == 0) *slot = static_cast<int>(pending);
+    __syncthreads();
+    const int next_one = *slot;
+    __syncthreads();
+    return next_one;
+  }
+  __device__ void leave() {
+    if (mode == ISP_BATCH_QUEUE_DYNAMIC && threadIdx.x == 0) {
+      const unsigned int ticket = atomicAdd(exited, 1U);
+      if (ticket == gridDim.x - 1) {
+        atomicExch(next, 0U);
+        atomicExch(exited, 0U);
+      }
+    }
+  }
+};
+
+// White balance statistics of every auto-balanced frame of a batch: the items
+// of all frames in one persistent grid; each frame's last contributor turns
+// its sums into the stream's gains, which the main pass (the next launch) reads.
+__global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* __restrict__ table, BatchControl* ctl) {
+  __shared__ AwbFrameLds fl;
+  __shared__ __attribute__((aligned(16))) AwbTileLds tl;
+  __shared__ int slot;
+  BatchQueue q{&ctl->awb_next, &ctl->awb_exited, table->awb_items, table->queue, &slot};
+  const int count = table->count;
+  int cur = -1;
   unsigned int items_done = 0;
   unsigned long long acc[4] = {0, 0, 0, 0};
   AwbCtx ac;
-  MainCtx mc;
-  __syncthreads();
-  const unsigned int ready_value = static_cast<unsigned int>(slot[1]) + 1U;
-
-  // Closes the block's contribution to frame `cur`; the last contributor runs
-  // the frame's epilogue.
-  auto close = [&]() {
-    const BatchFrame& b = table->frames[cur];
-    if (cur_pass == 0) {
-      const unsigned int token = awb_flush(&b.s->awb, afl, ac, acc);
-      if (complete(&b.s->awb.done, items_done, static_cast<unsigned int>(b.awb_items), token, &afl.flag)) {
-        awb_finish(b.f, b.s, afl, ac, &ctl->ready[cur], ready_value);
-      }
-    } else {
-      const unsigned int token = main_flush(&b.s->main, mfl, mc, acc);
-      if (complete(&b.s->main.done, items_done, static_cast<unsigned int>(b.items), token, &mfl.flag)) {
-        main_finish(b.f, &b.s->main, mfl, mc);
-      }
-    }
-    __syncthreads();  // the epilogue is done with the frame LDS
-#pragma unroll
-    for (int k = 0; k < 4; ++k) acc[k] = 0;
-    items_done = 0;
-  };
-
-  for (;;) {
-    const int item = slot[0];
-    __syncthreads();  // every thread has the item before thread 0 replaces it
-    if (item >= total) break;
-    if (tid == 0) pending = atomicAdd(&ctl->next, 1U);  // the next item, in flight meanwhile
-    const int pass = item < awb_items ? 0 : 1;
+  for (int item = q.first(); item < q.total; item = q.next_item(item)) {
+    q.prefetch();
     int local = 0;
-    const int f = pass == 0 ? batch_locate(table->awb_start, table->awb_order, count, item, &local)
-                            : batch_locate(table->main_start, table->main_order, count, item - awb_items, &local);
-    if (f != cur || pass != cur_pass) {
-      if (cur >= 0) close();
-      cur = f;
-      cur_pass = pass;
-      const BatchFrame& b = table->frames[f];
-      if (pass == 0) {
-        awb_begin(b.f, afl, ac);
-      } else {
-        float gains[3];
-        if (b.f.wb_mode != ISP_WB_MANUAL) {
-          if (tid == 0) {
-            // Bounded wait for this frame's white balance (see the file comment).
-            unsigned int rounds = 0;
-            while (atomicOr(&ctl->ready[f], 0U) != ready_value) {
-              if (++rounds > spin_limit) {
-                atomicOr(&ctl->error, 1U);
-                break;
-              }
-              __builtin_amdgcn_s_sleep(8);
-            }
-            for (int k = 0; k < 3; ++k) mfl.gains[k] = coherent_read(&b.s->wb.gains[k]);
-          }
-          __syncthreads();
-#pragma unroll
-          for (int k = 0; k < 3; ++k) gains[k] = mfl.gains[k];
-        } else {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) gains[k] = b.f.wb[k];
-        }
-        main_begin(b.f, gains, mfl, mc);
-      }
-    }
+    const int f = batch_locate(table->awb_start, table->awb_order, count, item, &local);
     const BatchFrame& b = table->frames[f];
-    if (pass == 0) {
-      switch (b.f.raw_format) {
-        case ISP_RAW_U8:
-          batch_awb_item<ISP_RAW_U8>(b, ac, afl, tile_lds, local, acc);
-          break;
-        case ISP_RAW_U16:
-          batch_awb_item<ISP_RAW_U16>(b, ac, afl, tile_lds, local, acc);
-          break;
-        case ISP_RAW_RAW10:
-          batch_awb_item<ISP_RAW_RAW10>(b, ac, afl, tile_lds, local, acc);
-          break;
-        default:
-          batch_awb_item<ISP_RAW_RAW12>(b, ac, afl, tile_lds, local, acc);
-          break;
+    if (f != cur) {
+      if (cur >= 0) {
+        const BatchFrame& pb = table->frames[cur];
+        const unsigned int token = awb_flush(&pb.s->awb, fl, ac, acc);
+        if (complete(&pb.s->awb.done, items_done, static_cast<unsigned int>(pb.awb_items), token, &fl.flag)) {
+          awb_finish(pb.f, pb.s, fl, ac);
+        }
+        __syncthreads();  // the epilogue is done with the frame LDS
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] = 0;
+        items_done = 0;
       }
-    } else {
-      switch (b.variant) {
-        ISP_BATCH_MAIN_CASES(ISP_RAW_U8)
-        ISP_BATCH_MAIN_CASES(ISP_RAW_U16)
-        ISP_BATCH_MAIN_CASES(ISP_RAW_RAW10)
-        ISP_BATCH_MAIN_CASES(ISP_RAW_RAW12)
-        default:
-          break;
-      }
+      cur = f;
+      awb_begin(b.f, fl, ac);
+    }
+    switch (b.f.raw_format) {
+      case ISP_RAW_U8:
+        batch_awb_item<ISP_RAW_U8>(b, ac, fl, tl, local, acc);
+        break;
+      case ISP_RAW_U16:
+        batch_awb_item<ISP_RAW_U16>(b, ac, fl, tl, local, acc);
+        break;
+      case ISP_RAW_RAW10:
+        batch_awb_item<ISP_RAW_RAW10>(b, ac, fl, tl, local, acc);
+        break;
+      default:
+        batch_awb_item<ISP_RAW_RAW12>(b, ac, fl, tl, local, acc);
+        break;
     }
     ++items_done;
-    if (tid == 0) slot[0] = static_cast<int>(pending);
-    __syncthreads();
   }
-  if (cur >= 0) close();
-
-  // The last block out leaves the queue ready for the next launch.
-  if (tid == 0) {
-    const unsigned int ticket = atomicAdd(&ctl->exited, 1U);
-    if (ticket == gridDim.x - 1) {
-      atomicExch(&ctl->next, 0U);
-      atomicExch(&ctl->exited, 0U);
-      atomicAdd(&ctl->tick, 1U);
+  if (cur >= 0) {
+    const BatchFrame& pb = table->frames[cur];
+    const unsigned int token = awb_flush(&pb.s->awb, fl, ac, acc);
+    if (complete(&pb.s->awb.done, items_done, static_cast<unsigned int>(pb.awb_items), token, &fl.flag)) {
+      awb_finish(pb.f, pb.s, fl, ac);
     }
   }
+  q.leave();
+}
+
+#define ISP_BATCH_MAIN_CASE(FMT, MAL, SHP, TH)                                                            \
+  case variant_of(FMT, MAL, SHP, TH):                                                                     \
+    if constexpr (kSharpen || !SHP) batch_main_item<FMT, MAL, SHP, TH>(b, mc, fl, tile_lds, local, acc); \
+    break;
+#define ISP_BATCH_MAIN_CASES(FMT)            \
+  ISP_BATCH_MAIN_CASE(FMT, false, false, 32) \
+  ISP_BATCH_MAIN_CASE(FMT, false, true, 32)  \
+  ISP_BATCH_MAIN_CASE(FMT, true, false, 32)  \
+  ISP_BATCH_MAIN_CASE(FMT, true, true, 32)   \
+  ISP_BATCH_MAIN_CASE(FMT, false, false, 16) \
+  ISP_BATCH_MAIN_CASE(FMT, false, true, 16)  \
+  ISP_BATCH_MAIN_CASE(FMT, true, false, 16)  \
+  ISP_BATCH_MAIN_CASE(FMT, true, true, 16)
+
+// The main pass of every frame of a batch in one persistent grid. kSharpen: the
+// batch has frames with sharpening (their tile bodies need the larger LDS); a
+// batch without them runs a lighter kernel at 3 blocks per CU.
+template <bool kSharpen>
+__device__ __forceinline__ void batch_main_body(const BatchTable* __restrict__ table, BatchControl* ctl) {
+  __shared__ MainFrameLds fl;
+  __shared__ __attribute__((aligned(16))) unsigned char tile_lds[batch_tile_lds<kSharpen>()];
+  __shared__ int slot;
+  BatchQueue q{&ctl->main_next, &ctl->main_exited, table->main_items, table->queue, &slot};
+  const int count = table->count;
+  int cur = -1;
+  unsigned int items_done = 0;
+  unsigned long long acc[4] = {0, 0, 0, 0};
+  MainCtx mc;
+  auto close = [&]() {
+    const BatchFrame& pb = table->frames[cur];
+    const unsigned int token = main_flush(&pb.s->main, fl, mc, acc);
+    if (complete(&pb.s->main.done, items_done, static_cast<unsigned int>(pb.items), token, &fl.flag)) {
+      main_finish(pb.f, &pb.s->main, fl, mc);
+    }
+  };
+  for (int item = q.first(); item < q.total; item = q.next_item(item)) {
+    q.prefetch();
+    int local = 0;
+    const int f = batch_locate(table->main_start, table->main_order, count, item, &local);
+    const BatchFrame& b = table->frames[f];
+    if (f != cur) {
+      if (cur >= 0) {
+        close();
+        __syncthreads();  // the epilogue is done with the frame LDS
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] = 0;
+        items_done = 0;
+      }
+      cur = f;
+      // The white balance pass of this batch ended with the previous launch.
+      float gains[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gains[k] = b.f.wb_mode != ISP_WB_MANUAL ? b.s->wb.gains[k] : b.f.wb[k];
+      main_begin(b.f, gains, fl, mc);
+    }
+    switch (b.variant) {
+      ISP_BATCH_MAIN_CASES(ISP_RAW_U8)
+      ISP_BATCH_MAIN_CASES(ISP_RAW_U16)
+      ISP_BATCH_MAIN_CASES(ISP_RAW_RAW10)
+      ISP_BATCH_MAIN_CASES(ISP_RAW_RAW12)
+      default:
+        break;
+    }
+    ++items_done;
+  }
+  if (cur >= 0) close();
+  q.leave();
 }
 
 #undef ISP_BATCH_MAIN_CASES
 #undef ISP_BATCH_MAIN_CASE
+
+__global__ void __launch_bounds__(kThreads) batch_main_kernel_sharpen(const BatchTable* __restrict__ table,
+                                                                      BatchControl* ctl) {
+  batch_main_body<true>(table, ctl);
+}
+
+// Three resident blocks per CU: the registers are held to what three waves per
+// SIMD leave (the LDS of this kernel fits three blocks).
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 3)))
+batch_main_kernel_plain(const BatchTable* __restrict__ table, BatchControl* ctl) {
+  batch_main_body<false>(table, ctl);
+}
 
 // ---- launches ---------------------------------------------------------------
 
@@ -1583,18 +1641,26 @@ void launch_main(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen
   }
 }
 
-namespace {
-std::atomic<unsigned int> g_spin_limit{1U << 24};  // ~ 8 * 64 clocks a round: about 4 s at 2.4 GHz
-}  // namespace
+int batch_main_blocks_per_cu(bool sharpen) {
+  return sharpen ? blocks_per_cu(batch_main_kernel_sharpen) : blocks_per_cu(batch_main_kernel_plain);
+}
 
-void set_batch_spin_limit(unsigned int rounds) { g_spin_limit = rounds; }
+int batch_awb_blocks_per_cu() { return std::min(4, blocks_per_cu(batch_awb_kernel)); }
 
-int batch_blocks_per_cu() { return blocks_per_cu(batch_kernel); }
+void launch_batch_awb(const BatchTable* table, BatchControl* control, int grid, void* stream) {
+  hipLaunchKernelGGL(batch_awb_kernel, dim3(grid), dim3(kThreads), 0, static_cast<hipStream_t>(stream), table,
+                     control);
+  check(hipGetLastError(), "batch_awb_kernel launch");
+}
 
-void launch_batch(const BatchTable* table, BatchControl* control, int grid, void* stream) {
-  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kThreads), 0, static_cast<hipStream_t>(stream), table, control,
-                     g_spin_limit.load());
-  check(hipGetLastError(), "batch_kernel launch");
+void launch_batch_main(const BatchTable* table, BatchControl* control, int grid, bool sharpen, void* stream) {
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (sharpen) {
+    hipLaunchKernelGGL(batch_main_kernel_sharpen, dim3(grid), dim3(kThreads), 0, st, table, control);
+  } else {
+    hipLaunchKernelGGL(batch_main_kernel_plain, dim3(grid), dim3(kThreads), 0, st, table, control);
+  }
+  check(hipGetLastError(), "batch_main_kernel launch");
 }
 
 }  // namespace isp_amd

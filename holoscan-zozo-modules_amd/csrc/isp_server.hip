@@ -8,7 +8,8 @@
 //     hold its buffers; the first request that does not fit stops admission for
 //     this round, so large frames are not starved by smaller ones behind them;
 //  3. copies the admitted host frames into the arena on the slot's stream and
-//     launches them as one batch (or one per-frame launch each, the baseline).
+//     launches them as one batch (a lone frame, or every frame in the baseline
+//     mode, as a per-frame launch).
 // Launch slots are independent streams, so a launch starts while earlier ones
 // still run. A camera is one pipeline (tables, white balance and exposure
 // state on the device); at most one of its frames is in flight, so its frames
@@ -184,17 +185,6 @@ struct isp_server {
       const int64_t t = now_ns();
       std::string error;
       if (q != hipSuccess) error = std::string("IspPipeline (hip): launch: ") + hipGetErrorString(q);
-      // A launch gives up its bounded wait for a white balance only after several
-      // seconds of waiting; only then is the flag worth a read.
-      if (error.empty() && !config.per_frame && !slot.frames.empty() &&
-          t - slot.frames.front()->result.launch_ns > 1000000000LL) {
-        int32_t expired = 0;
-        if (isp_batch_status(slot.batch, &expired) != ISP_OK) {
-          error = isp_last_error();
-        } else if (expired != 0) {
-          error = "IspPipeline (hip): a batched launch gave up waiting for a frame's white balance";
-        }
-      }
       for (Request* r : slot.frames) {
         r->result.done_ns = t;
         if (!error.empty()) r->result.status = ISP_FAILURE;
@@ -274,7 +264,9 @@ struct isp_server {
       r->result.batch_frames = static_cast<int32_t>(take.size());
     }
     std::string error;
-    if (config.per_frame) {
+    // A launch of one frame runs the per-frame kernels: for a lone frame they
+    // start faster than the batch kernels' work queue.
+    if (config.per_frame || frames.size() == 1) {
       for (const isp_batch_frame& f : frames) {
         if (isp_process(f.pipeline, &f.raw, &f.out, &f.stats, &f.exposure, slot.stream) != ISP_OK) {
           error = isp_last_error();
@@ -338,6 +330,7 @@ void isp_server_config_init(isp_server_config* config) {
   config->per_frame = 0;
   isp_batch_schedule_init(&config->schedule);
   config->schedule.graph = 0;  // every launch has a different frame set
+  config->schedule.queue = ISP_BATCH_QUEUE_STATIC;  // small launches: no queue atomics
 }
 
 isp_status isp_server_create(const isp_server_config* config, isp_server** out) {

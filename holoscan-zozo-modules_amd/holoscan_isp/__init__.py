@@ -295,11 +295,9 @@ class IspBatch:
     def launch(self, stream=None):
         check(lib().isp_batch_launch(self._h, stream), "batch launch")
 
-    def status(self):
-        """Waits for the launches; True if a launch gave up waiting for a white balance."""
-        flag = C.c_int32()
-        check(lib().isp_batch_status(self._h, C.byref(flag)), "batch status")
-        return bool(flag.value)
+    def synchronize(self):
+        """Waits for the batch's launches."""
+        check(lib().isp_batch_synchronize(self._h), "batch synchronize")
 
     def tune(self, stream=None, cache_path=None, launches=5):
         s = _abi.BatchSchedule()

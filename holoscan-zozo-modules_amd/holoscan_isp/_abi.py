@@ -55,6 +55,7 @@ class Params(C.Structure):
 
 BATCH_MAX_FRAMES = 64
 BATCH_ORDER_LARGEST_FIRST, BATCH_ORDER_AS_GIVEN = 0, 1
+BATCH_QUEUE_DYNAMIC, BATCH_QUEUE_STATIC, BATCH_QUEUE_STATIC_XCD = 0, 1, 2
 
 
 class RawView(C.Structure):
@@ -83,7 +84,7 @@ class BatchFrame(C.Structure):
 class BatchSchedule(C.Structure):
     _fields_ = [("tile_height", (C.c_int32 * 2) * 4), ("tiles_per_item", (C.c_int32 * 2) * 4),
                 ("awb_tiles_per_item", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("order", C.c_int32), ("graph", C.c_int32)]
+                ("order", C.c_int32), ("queue", C.c_int32), ("graph", C.c_int32)]
 
 
 class ServerConfig(C.Structure):
@@ -171,7 +172,7 @@ PROTOTYPES = [
     ("isp_batch_get_schedule", C.c_int, [_void, P(BatchSchedule)]),
     ("isp_batch_set_frames", C.c_int, [_void, P(BatchFrame), C.c_int32]),
     ("isp_batch_launch", C.c_int, [_void, _void]),
-    ("isp_batch_status", C.c_int, [_void, P(C.c_int32)]),
+    ("isp_batch_synchronize", C.c_int, [_void]),
     ("isp_batch_tune", C.c_int, [_void, _void, C.c_char_p, C.c_int32, P(BatchSchedule), P(C.c_double)]),
     ("isp_server_config_init", None, [P(ServerConfig)]),
     ("isp_server_create", C.c_int, [P(ServerConfig), P(_void)]),

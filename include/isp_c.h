@@ -241,7 +241,8 @@ isp_status isp_set_exposure_feedback(isp_pipeline* pipeline, int32_t enable, con
 isp_status isp_get_exposure_state(isp_pipeline* pipeline, isp_frame_exposure* state);
 
 /* ---- the batched form (no reference counterpart) -------------------------
- * Many camera streams in one persistent-kernel launch. Each frame of a batch
+ * Many camera streams in one persistent-kernel launch per pass (the white
+ * balance statistics of the auto-balanced frames, then the main pass of all). Each frame of a batch
  * belongs to a pipeline (its stream: parameters, tables, white balance and
  * exposure state), at most one frame per pipeline, and its results are the
  * ones isp_process gives for that pipeline. Frames of any raw format, size,
@@ -260,6 +261,10 @@ typedef struct isp_batch_frame {
 } isp_batch_frame;
 
 enum { ISP_BATCH_ORDER_LARGEST_FIRST = 0, ISP_BATCH_ORDER_AS_GIVEN = 1 };
+/* How the blocks of a launch share its work items: an atomic queue (dynamic
+ * balance), a fixed round robin, or a fixed split into one contiguous range per
+ * XCD (neighbouring tiles meet in one L2). */
+enum { ISP_BATCH_QUEUE_DYNAMIC = 0, ISP_BATCH_QUEUE_STATIC = 1, ISP_BATCH_QUEUE_STATIC_XCD = 2 };
 
 /* How a batch runs. The per-format schedule picks the main pass's tile height
  * and work-item size for each raw format and demosaic; the rest applies to the
@@ -270,11 +275,12 @@ typedef struct isp_batch_schedule {
   int32_t awb_tiles_per_item; /* white-balance 64 x 64 tiles per work item, 1..64 */
   int32_t blocks_per_cu;      /* resident blocks per CU; 0 = as many as fit */
   int32_t order;              /* ISP_BATCH_ORDER_*: which frames' work the queue lists first */
-  int32_t graph;              /* 1: capture the launch as a hipGraph once and replay it */
+  int32_t queue;              /* ISP_BATCH_QUEUE_* */
+  int32_t graph;              /* 1: capture the launches as a hipGraph once and replay it */
 } isp_batch_schedule;
 
 /* The default schedule: 32-row tiles, 4 tiles per item, 8 white-balance tiles
- * per item, full occupancy, largest frames first, graph replay. */
+ * per item, full occupancy, largest frames first, the dynamic queue, graph replay. */
 void isp_batch_schedule_init(isp_batch_schedule* schedule);
 isp_status isp_batch_create(isp_batch** out);
 /* Waits for the batch's launches, then frees. */
@@ -288,9 +294,8 @@ isp_status isp_batch_set_frames(isp_batch* batch, const isp_batch_frame* frames,
 /* Queues one launch of the bound frames on `stream`, after every member
  * pipeline's queued frames; each pipeline's later frames wait for it. */
 isp_status isp_batch_launch(isp_batch* batch, isp_stream_t stream);
-/* Waits for the batch's launches; *wait_expired = 1 if a launch gave up waiting
- * for a frame's white balance (its results are then not valid), and clears it. */
-isp_status isp_batch_status(isp_batch* batch, int32_t* wait_expired);
+/* Waits for the batch's launches. */
+isp_status isp_batch_synchronize(isp_batch* batch);
 /* Autotuner: times candidate schedules on the bound frames (`launches` timed
  * launches each) and keeps the fastest; with `cache_path`, a schedule found for
  * the same device and frame set is reused without timing, and new results are
@@ -357,7 +362,9 @@ typedef struct isp_server_counters {
   int32_t max_in_flight; /* the most launches ever on the GPU at once */
 } isp_server_counters;
 
-/* 256 MiB budget, 32 frames per launch, 2 launch slots, batched, the default schedule. */
+/* 256 MiB budget, 32 frames per launch, 2 launch slots, batched, the default
+ * schedule with the static queue and no graph (every launch differs). A launch
+ * of a single frame runs the per-frame kernels. */
 void isp_server_config_init(isp_server_config* config);
 isp_status isp_server_create(const isp_server_config* config, isp_server** out);
 /* Finishes every submitted frame, then frees everything. */

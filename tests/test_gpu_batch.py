@@ -106,7 +106,7 @@ def run_batch(specs, schedule=None, ticks=1, between=None, feedback=None):
             between()
         torch.cuda.synchronize()
         results.append([s.result() for s in slots])
-    assert not batch.status()
+    batch.synchronize()
     return results
 
 
@@ -153,6 +153,8 @@ def test_batch_matches_per_frame_and_oracle(torch_gpu):
 
 SCHEDULES = [
     dict(graph=0),
+    dict(queue=_abi.BATCH_QUEUE_STATIC),
+    dict(queue=_abi.BATCH_QUEUE_STATIC_XCD, tiles_per_item=1, tile16=True),
     dict(tile16=True),
     dict(tiles_per_item=1, awb_tiles_per_item=1),
     dict(tiles_per_item=32, awb_tiles_per_item=64, order=_abi.BATCH_ORDER_AS_GIVEN),
