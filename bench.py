@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=6, help="frames of the single-thread CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--batch-ticks", type=int, default=100, help="timed ticks of the 16-stream batch (0: skip)")
+    ap.add_argument("--large-ticks", type=int, default=3, help="timed batches of 8 x 8192x6144 frames (0: skip)")
     ap.add_argument("--served-seconds", type=float, default=2.0,
                     help="real-time length of the 32-camera serving trace (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -187,6 +188,61 @@ def batched_vs_per_frame(ticks):
                         "14-bit; gray-world, bilinear/Malvar alternating, RGB8 + histograms + stats",
             "pixels_per_tick": pixels, "ticks": ticks, "per_frame_launches": single, "batched": batched,
             "speedup": round(single["ms_per_tick"] / batched["ms_per_tick"], 3)}
+
+
+def large_frames(frames, ticks):
+    """The large-frame batch: `frames` streams of 8192x6144 RGGB u16 at 16 bits (BL 1024, white
+    patch, Malvar, sharpening, 65536-entry tone table, RGB16 + 1024-bin histograms), as per-frame
+    launches and as one batch. One scene is rendered and copied to every stream's buffer."""
+    import torch
+    import holoscan_isp as isp
+    import cases
+    sc, params, out_format = cases.config5()
+    frame = isp.render_scene(sc)
+    W, H = frame.width, frame.height
+    d_raw = [torch.from_numpy(frame.data).to("cuda") for _ in range(frames)]
+    pitch = W * 6
+    outs = [torch.empty(pitch * H, dtype=torch.uint8, device="cuda") for _ in range(frames)]
+    hists = [torch.empty(4 * 1024, dtype=torch.int32, device="cuda") for _ in range(frames)]
+    stats = [torch.empty(12, dtype=torch.float32, device="cuda") for _ in range(frames)]
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    exposure = (10_000_000, 2.0)
+    pixels = frames * W * H
+    algo = frames * W * H * (2 + 6)
+
+    def timed(step):
+        step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(ticks):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / ticks
+        return {"ms_per_batch": round(ms, 3), "gpix_s": round(pixels / ms / 1e6, 3),
+                "hbm_gbs_algorithmic": round(algo / ms / 1e6, 1)}
+
+    pipes = [isp.IspPipeline(params) for _ in range(frames)]
+    views = [frame.view(d) for d in d_raw]
+
+    def per_frame():
+        for i in range(frames):
+            pipes[i].process(views[i], outs[i], pitch, out_format, hists[i], stats[i], exposure, sh)
+
+    single = timed(per_frame)
+    bpipes = [isp.IspPipeline(params) for _ in range(frames)]
+    batch = isp.IspBatch()
+    batch.set_frames([(bpipes[i], views[i], outs[i], pitch, out_format, hists[i], stats[i], exposure)
+                      for i in range(frames)])
+    chosen, _ = batch.tune(sh, None, 2)
+    batched = timed(lambda: batch.launch(sh))
+    batch.synchronize()
+    return {"workload": f"{frames} x 8192x6144 RGGB u16 16-bit, BL 1024, white patch, Malvar, sharpen 0.5, CCM, "
+                        "65536-entry sRGB table, RGB16 + 4x1024 histograms + stats",
+            "algorithmic_bytes_per_batch": algo, "per_frame_launches": single, "batched": batched,
+            "batched_queue": chosen.queue}
 
 
 def served(seconds, per_frame, device_source, saturate=False, fps=30.0):
@@ -389,6 +445,8 @@ def main():
     }
     if args.batch_ticks > 0:
         result["batched_16_streams"] = batched_vs_per_frame(args.batch_ticks)
+    if args.large_ticks > 0:
+        result["large_frames_8"] = large_frames(8, args.large_ticks)
     if args.served_seconds > 0:
         result["served_32_cameras"] = served_report(args.served_seconds)
     if not args.no_cpu:

@@ -111,6 +111,58 @@ class ServerCounters(C.Structure):
                 ("max_in_flight", C.c_int32)]
 
 
+PACKING_NONE, PACKING_MIPI_RAW10, PACKING_MIPI_RAW12 = 0, 1, 2
+MEMORY_PAGEABLE, MEMORY_PINNED, MEMORY_DEVICE = 0, 1, 2
+DTYPE_INT, DTYPE_UINT, DTYPE_FLOAT = 0, 1, 2
+(ENCODING_UNKNOWN, ENCODING_RGB8, ENCODING_RGBA8, ENCODING_RGB16, ENCODING_RGB32F, ENCODING_BAYER_RGGB,
+ ENCODING_BAYER_BGGR, ENCODING_BAYER_GRBG, ENCODING_BAYER_GBRG, ENCODING_MONO8, ENCODING_MONO16) = range(11)
+COLOR_SPACE_UNSPECIFIED, COLOR_SPACE_SRGB = 0, 1
+UNKNOWN_SIGNIFICANT_BITS = 255
+
+
+class Tensor(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("memory", C.c_uint8), ("dtype_code", C.c_uint8),
+                ("dtype_bits", C.c_uint8), ("ndim", C.c_int32), ("shape", C.c_int64 * 4),
+                ("strides", C.c_int64 * 4), ("ready", C.c_void_p)]
+
+
+class Image(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("encoding", C.c_uint8),
+                ("significant_bits", C.c_uint8), ("color_space", C.c_uint8), ("roi_offset_x", C.c_int32),
+                ("roi_offset_y", C.c_int32), ("exposure_time_ns", C.c_int64), ("gain", C.c_float),
+                ("header", C.c_void_p), ("data", Tensor)]
+
+
+class SampleMetadata(C.Structure):
+    _fields_ = [("capture_timestamp_ns", C.c_int64), ("frame_id", C.c_uint64)]
+
+
+class OpConfig(C.Structure):
+    _fields_ = [("backend", C.c_uint8), ("params", Params), ("output_format", C.c_uint8),
+                ("has_pattern", C.c_uint8), ("pattern", C.c_uint8), ("bit_depth", C.c_int32),
+                ("packing", C.c_uint8), ("stats", C.c_uint8), ("max_width", C.c_int32),
+                ("max_height", C.c_int32)]
+
+
+class OpPort(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("is_input", C.c_uint8), ("queue_depth", C.c_uint32),
+                ("memory", C.c_uint8), ("dtype_code", C.c_uint8), ("dtype_bits", C.c_uint8),
+                ("rank", C.c_int32), ("max_bytes", C.c_uint64)]
+
+
+class OpStatistics(C.Structure):
+    _fields_ = [("host_waits", C.c_uint64), ("copies", C.c_uint64)]
+
+
+ALLOCATE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.POINTER(Tensor), C.POINTER(Tensor))
+EMIT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.POINTER(Image), C.POINTER(Tensor),
+                      C.POINTER(SampleMetadata))
+
+
+class OpRuntime(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("allocate", ALLOCATE_FN), ("emit", EMIT_FN)]
+
+
 class SyntheticConfig(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("pattern", C.c_uint8),
                 ("format", C.c_uint8), ("bit_depth", C.c_int32),
@@ -183,6 +235,16 @@ PROTOTYPES = [
     ("isp_server_release", C.c_int, [_void, C.c_int64]),
     ("isp_server_counters_get", C.c_int, [_void, P(ServerCounters)]),
     ("isp_server_now_ns", C.c_int64, []),
+    ("isp_op_config_init", None, [P(OpConfig)]),
+    ("isp_op_validate_config", C.c_int, [P(OpConfig)]),
+    ("isp_op_create", C.c_int, [P(OpConfig), P(_void)]),
+    ("isp_op_destroy", C.c_int, [_void]),
+    ("isp_op_backend", C.c_uint8, [_void]),
+    ("isp_op_ports", C.c_int, [_void, P(OpPort), C.c_int32, P(C.c_int32)]),
+    ("isp_op_start", C.c_int, [_void]),
+    ("isp_op_stop", C.c_int, [_void]),
+    ("isp_op_compute", C.c_int, [_void, P(Image), P(SampleMetadata), _void, P(OpRuntime)]),
+    ("isp_op_statistics", OpStatistics, []),
     ("isp_synthetic_config_init", None, [P(SyntheticConfig)]),
     ("isp_synthetic_render", C.c_int, [P(SyntheticConfig), _void, C.c_size_t]),
     ("isp_synthetic_params", C.c_int, [P(SyntheticConfig), C.c_int32, P(Params), P(C.c_float)]),

@@ -42,11 +42,15 @@ STRUCTS = {"isp_params": _abi.Params, "isp_lens_shading": _abi.LensShading,
            "isp_synthetic_config": _abi.SyntheticConfig, "isp_scene_config": _abi.SceneConfig,
            "isp_batch_frame": _abi.BatchFrame, "isp_batch_schedule": _abi.BatchSchedule,
            "isp_server_config": _abi.ServerConfig, "isp_camera_config": _abi.CameraConfig,
-           "isp_server_result": _abi.ServerResult, "isp_server_counters": _abi.ServerCounters}
+           "isp_server_result": _abi.ServerResult, "isp_server_counters": _abi.ServerCounters,
+           "isp_tensor": _abi.Tensor, "isp_image": _abi.Image, "isp_sample_metadata": _abi.SampleMetadata,
+           "isp_op_config": _abi.OpConfig, "isp_op_port": _abi.OpPort, "isp_op_statistics_t": _abi.OpStatistics,
+           "isp_op_runtime": _abi.OpRuntime}
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
-    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "isp_synthetic.h"', 'int main(void) {']
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "isp_synthetic.h"', '#include "isp_op.h"',
+           'int main(void) {']
     for cname, py in STRUCTS.items():
         src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for field, _ in py._fields_:

@@ -144,6 +144,19 @@ def test_baseline_configs_full_size(torch_gpu, which):
                 cpu.process(frame, out_format, True, EXPOSURE, threads=16))
 
 
+def test_config5_full_size(torch_gpu):
+    """The large-frame configuration at full size: 8192x6144 u16 at 16 bits, white patch, Malvar,
+    sharpening, a 65536-entry tone table (read from global memory), RGB16 and 1024-bin
+    histograms; one frame against the oracle, and the histogram rebuilt from the image."""
+    from test_oracle import rebuild_histogram
+    sc, params, out_format = cases.config5()
+    frame = isp.render_scene(sc)
+    g = gs.run(isp.IspPipeline(params), frame, out_format, True, EXPOSURE)
+    compare("config5", g, OraclePipeline(params).process(frame, out_format, True, EXPOSURE, threads=16))
+    assert np.array_equal(g.histogram, rebuild_histogram(g.image, out_format, frame.width, frame.height, 1024))
+    assert all(int(row.sum()) == frame.width * frame.height for row in g.histogram.reshape(4, 1024))
+
+
 def test_determinism(torch_gpu):
     """determinism (pipeline_test.cpp:621-634): same bytes on every run and instance."""
     config = cases.camera(_abi.RGGB, cases.FORMATS[2], 640, 480)
