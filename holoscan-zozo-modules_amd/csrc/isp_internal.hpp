@@ -95,23 +95,29 @@ struct FrameDesc {
 };
 
 // Per-pipeline device scratch, zero between frames (each pass's last block clears
-// what it consumed).
+// what it consumed). The blocks of a launch add into kSlots copies of every
+// accumulator, block b into copy b % kSlots, so that no address takes more than
+// a kSlots-th of the blocks' device-scope atomics (same-address atomics
+// serialise where they are performed); the last block adds the copies up.
+inline constexpr int kSlots = 8;
+
 struct AwbScratch {
-  unsigned long long sum[3];
-  unsigned long long quads;
-  unsigned int hist[3][kAwbBins];
-  unsigned int done;  // blocks finished
-  unsigned int pad;
+  unsigned long long sum[kSlots][4];  // R, G, B fixed sums; complete quads
+  unsigned int hist[kSlots][3][kAwbBins];
+  unsigned int done[kSlots];  // blocks finished, per slot
+  unsigned int top;           // slots finished
+  unsigned int pad[7];
 };
 struct WbState {
   float gains[3];
   int valid;
 };
 struct MainScratch {
-  unsigned long long sum[4];  // R, G, B fixed sums; saturated samples
-  unsigned int hist[4][kMaxHistBins];
-  unsigned int done;
-  unsigned int pad;
+  unsigned long long sum[kSlots][4];  // R, G, B fixed sums; saturated samples
+  unsigned int hist[kSlots][4][kMaxHistBins];
+  unsigned int done[kSlots];
+  unsigned int top;
+  unsigned int pad[7];
 };
 struct DeviceScratch {
   AwbScratch awb;

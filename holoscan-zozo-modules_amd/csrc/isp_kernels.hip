@@ -148,13 +148,14 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // partial sums with device-scope atomics whose results each thread consumes
 // (folded into `token`), so every atomic has been performed at the point of
 // coherence before the thread reaches the barrier; only then does one thread
-// add the block's `count` contributions to the counter. The block whose
-// addition reaches `total` is the last one: it reads the totals back with
-// atomic exchanges, which also clear them for the next frame. (An agent-scope
-// fence here would write back and invalidate the whole L2 of the block's XCD.)
-// `flag` is a block-shared int.
-__device__ __forceinline__ bool complete(unsigned int* done, unsigned int count, unsigned int total,
-                                         unsigned int token, int* flag) {
+// count the block's contribution. The block that completes the count is the
+// last one: it reads the totals back and clears them for the next frame. (An
+// agent-scope fence here would write back and invalidate the whole L2 of the
+// block's XCD.) `flag` is a block-shared int.
+//
+// complete_items: `count` of `total` contributions counted on one counter.
+__device__ __forceinline__ bool complete_items(unsigned int* done, unsigned int count, unsigned int total,
+                                               unsigned int token, int* flag) {
   // Consuming the token makes the atomics return their values and the wave wait
   // for them here (the asm reads a register and touches no memory).
   __asm__ volatile("" : : "v"(token));
@@ -165,6 +166,41 @@ __device__ __forceinline__ bool complete(unsigned int* done, unsigned int count,
   }
   __syncthreads();
   return *flag != 0;
+}
+
+// complete_grid: one contribution per block of the grid, counted in two levels
+// so that no counter takes more than a kSlots-th of the blocks: block b counts
+// on done[b % kSlots]; the block that completes its slot's group (whose size
+// the grid fixes) counts on `top`; the one that completes `top` is the last.
+__device__ __forceinline__ bool complete_grid(unsigned int* done, unsigned int* top, unsigned int token, int* flag) {
+  __asm__ volatile("" : : "v"(token));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int g = gridDim.x;
+    const unsigned int x = blockIdx.x % kSlots;
+    const unsigned int groups = g < kSlots ? g : kSlots;
+    const unsigned int members = (g - x + kSlots - 1) / kSlots;
+    int last = 0;
+    if (atomicAdd(&done[x], 1U) + 1U == members) last = atomicAdd(top, 1U) + 1U == groups ? 1 : 0;
+    *flag = last;
+  }
+  __syncthreads();
+  return *flag != 0;
+}
+
+// The last block's reset of complete_grid's counters (every block has counted).
+__device__ __forceinline__ void reset_grid(unsigned int* done, unsigned int* top) {
+  if (threadIdx.x < kSlots) atomicExch(&done[threadIdx.x], 0U);
+  if (threadIdx.x == 0) atomicExch(top, 0U);
+}
+
+// Reads and clears the kSlots copies of an accumulator, returning their sum.
+template <typename T>
+__device__ __forceinline__ T take_slots(T* first, std::size_t stride) {
+  T total = 0;
+#pragma unroll
+  for (int x = 0; x < kSlots; ++x) total += atomicExch(first + x * stride, T(0));
+  return total;
 }
 
 __device__ __forceinline__ unsigned int fold(unsigned long long v) {
@@ -507,7 +543,7 @@ __device__ __forceinline__ void awb_tile(const FrameDesc& f, const AwbCtx& c, Aw
 }
 
 // Adds the block's partial sums and histograms to the frame's scratch; returns
-// the token of the atomics (see complete()).
+// the token of the atomics (see complete_grid()).
 __device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl, const AwbCtx& c,
                                                   const unsigned long long local[4]) {
   const int tid = threadIdx.x;
@@ -522,11 +558,12 @@ __device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl
   if (tid < 4) {
     unsigned long long total = 0;
     for (int w = 0; w < kThreads / 64; ++w) total += fl.part[w][tid];
-    if (total != 0) token ^= fold(atomicAdd(tid < 3 ? &a->sum[tid] : &a->quads, total));
+    if (total != 0) token ^= fold(atomicAdd(&a->sum[blockIdx.x % kSlots][tid], total));
   }
   if (c.patch) {
+    const int x = blockIdx.x % kSlots;
     for (int i = tid; i < 3 * kAwbBins; i += kThreads) {
-      if (fl.hist[i] != 0U) token ^= atomicAdd(&a->hist[i / kAwbBins][i % kAwbBins], fl.hist[i]);
+      if (fl.hist[i] != 0U) token ^= atomicAdd(&a->hist[x][i / kAwbBins][i % kAwbBins], fl.hist[i]);
     }
   }
   return token;
@@ -536,9 +573,11 @@ __device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl
 // into the stream's white balance state; the reads clear the scratch.
 __device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s, AwbFrameLds& fl, const AwbCtx& c) {
   const int tid = threadIdx.x;
-  if (tid < 4) fl.sums[tid] = atomicExch(tid < 3 ? &s->awb.sum[tid] : &s->awb.quads, 0ULL);
+  if (tid < 4) fl.sums[tid] = take_slots(&s->awb.sum[0][tid], 4);
   if (c.patch) {
-    for (int i = tid; i < 3 * kAwbBins; i += kThreads) fl.hist[i] = atomicExch(&s->awb.hist[i / kAwbBins][i % kAwbBins], 0U);
+    for (int i = tid; i < 3 * kAwbBins; i += kThreads) {
+      fl.hist[i] = take_slots(&s->awb.hist[0][i / kAwbBins][i % kAwbBins], 3 * kAwbBins);
+    }
   }
   __syncthreads();
   const unsigned long long quads_s = fl.sums[3];
@@ -577,8 +616,7 @@ __device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s,
         token ^= atomicExch(reinterpret_cast<unsigned int*>(&st.valid), 1U);
       }
     }
-    __asm__ volatile("" : : "v"(token));  // the state is written before the ticket clears
-    atomicExch(&s->awb.done, 0U);
+    __asm__ volatile("" : : "v"(token));  // the state is written before the counters clear
   }
 }
 
@@ -592,8 +630,9 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
   unsigned long long local[4] = {0, 0, 0, 0};
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) awb_tile<kFmt>(f, c, fl, tl, tile, tiles_x, fast_rows, local);
   const unsigned int token = awb_flush(&s->awb, fl, c, local);
-  if (!complete(&s->awb.done, 1U, gridDim.x, token, &fl.flag)) return;
+  if (!complete_grid(s->awb.done, &s->awb.top, token, &fl.flag)) return;
   awb_finish(f, s, fl, c);
+  reset_grid(s->awb.done, &s->awb.top);
 }
 
 // ---- the main pass ----------------------------------------------------------
@@ -679,7 +718,7 @@ __device__ __forceinline__ float tone(const float* lut, int n1, float top, float
 
 constexpr int kLdsLut = 4100;  // tone tables up to this size are read from LDS
 
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
 struct MainGeom {
   static constexpr int kR = kMalvar ? 2 : 1;
   static constexpr int kH = kR + (kSharpen ? 1 : 0);
@@ -692,17 +731,20 @@ struct MainGeom {
   // Row chunks from x0 - kC cover the sample columns.
   static constexpr int kNC = (kSW - kPad + kC + kC - 1) / kC;
   static constexpr int kTasks = kSH * kNC;
-  static constexpr int kTPT = (kTasks + kThreads - 1) / kThreads;
+  static constexpr int kTPT = (kTasks + kBlock - 1) / kBlock;
   // Ring of demosaiced RGB: ring column rc = frame x - x0 + 2, row rr = y - y0 + 1;
-  // 9 blocks of 8 columns, one block row of 2 per thread.
+  // blocks of kRBW columns and 2 rows, one per thread (8 columns for 256-thread
+  // blocks, 4 for 512).
   static constexpr int kRW = 72;
   static constexpr int kRH = kSharpen ? kTileH + 2 : 2;
-  static constexpr int kRingBlocks = (kRW / 8) * (kRH / 2);
+  static constexpr int kRBW = kBlock >= 512 ? 4 : 8;
+  static constexpr int kRingBlocks = (kRW / kRBW) * (kRH / 2);
   // Output rows per thread (a 4 x kRows3 block).
-  static constexpr int kRows3 = kTileH / 16;
+  static constexpr int kRows3 = kTileH / (kBlock / 16);
   static constexpr int kLscCols = kNC * kC;
-  static_assert(kRingBlocks <= kThreads || !kSharpen, "one ring block per thread");
+  static_assert(kRingBlocks <= kBlock || !kSharpen, "one ring block per thread");
   static_assert(kTileH == 16 || kTileH == 32, "tile heights");
+  static_assert(kRows3 >= 1, "a row per thread at least");
 };
 
 // LDS of the main pass: per frame (tone table, histograms, lens shading grid,
@@ -711,14 +753,14 @@ struct MainFrameLds {
   float lut[kLdsLut];
   unsigned int hist[4 * kMaxHistBins];
   float grid[kMaxGrid];
-  unsigned long long part[kThreads / 64][4];
+  unsigned long long part[8][4];  // per wave, up to 512 threads
   unsigned long long sums[4];
   float gains[3];
   int flag;
 };
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
 struct MainTileLds {
-  using G = MainGeom<kFmt, kMalvar, kSharpen, kTileH>;
+  using G = MainGeom<kFmt, kMalvar, kSharpen, kTileH, kBlock>;
   alignas(16) float samp[G::kSH * G::kSW];
   alignas(16) float ring[3][G::kRH * G::kRW];
   alignas(16) LscTable<G::kLscCols> lsc;
@@ -742,6 +784,7 @@ struct MainCtx {
 
 // Starts a block's work on a frame. The tile's first barrier orders these LDS
 // writes before any reader.
+template <int kBlock = kThreads>
 __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains[3], MainFrameLds& fl, MainCtx& c) {
   const int tid = threadIdx.x;
   c.want_hist = f.histogram != nullptr;
@@ -753,11 +796,11 @@ __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains
   for (int k = 0; k < 3; ++k) c.gains[k] = gains[k];
   c.qc = quad_consts(f, gains);
   if (c.want_hist) {
-    for (int i = tid; i < 4 * kMaxHistBins; i += kThreads) fl.hist[i] = 0U;
+    for (int i = tid; i < 4 * kMaxHistBins; i += kBlock) fl.hist[i] = 0U;
   }
   c.lut_lds = f.lut_size <= kLdsLut;
   if (c.lut_lds) {
-    for (int i = tid; i < f.lut_size; i += kThreads) fl.lut[i] = f.lut[i];
+    for (int i = tid; i < f.lut_size; i += kBlock) fl.lut[i] = f.lut[i];
   }
   c.n1 = f.lut_size - 1;
   c.lut_top = f.lut[c.n1];
@@ -767,11 +810,11 @@ __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains
 
 // One kTileH x 64 output tile; statistics sums into acc[0..2], saturated
 // samples into acc[3].
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
 __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_, MainFrameLds& fl,
-                                          MainTileLds<kFmt, kMalvar, kSharpen, kTileH>& tl, int tile, int tiles_x,
-                                          int fast_rows, int out_aligned, unsigned long long acc[4]) {
-  using G = MainGeom<kFmt, kMalvar, kSharpen, kTileH>;
+                                          MainTileLds<kFmt, kMalvar, kSharpen, kTileH, kBlock>& tl, int tile,
+                                          int tiles_x, int fast_rows, int out_aligned, unsigned long long acc[4]) {
+  using G = MainGeom<kFmt, kMalvar, kSharpen, kTileH, kBlock>;
   constexpr int kR = G::kR;
   constexpr int kH = G::kH;
   constexpr int kC = G::kC;
@@ -803,7 +846,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
   Packed<kFmt> pk[kTPT];
 #pragma unroll
   for (int t = 0; t < kTPT; ++t) {
-    const int task = tid + t * kThreads;
+    const int task = tid + t * kBlock;
     if (task >= kTasks) break;
     const int sr = task / kNC;
     const int ci = task - sr * kNC;
@@ -827,7 +870,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
   unsigned int saturated = 0;
 #pragma unroll
   for (int t = 0; t < kTPT; ++t) {
-    const int task = tid + t * kThreads;
+    const int task = tid + t * kBlock;
     if (task >= kTasks) break;
     const int sr = task / kNC;
     const int ci = task - sr * kNC;
@@ -887,7 +930,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
   if (!interior_x) {
     // Columns outside the frame take the sample of their reflection.
     __syncthreads();
-    for (int i = tid; i < kSH * kSW; i += kThreads) {
+    for (int i = tid; i < kSH * kSW; i += kBlock) {
       const int sr = i / kSW;
       const int col = i - sr * kSW;
       const int x = x0 - kPad + col;
@@ -902,17 +945,19 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
   // (ring column rc = x - x0 + 2, row rr = y - y0 + 1), 8 x 2 pixels per thread.
   if constexpr (kSharpen) {
     if (tid < G::kRingBlocks) {
-      const int by = tid / (kRW / 8);
-      const int bx = tid - by * (kRW / 8);
-      const int rc0 = 8 * bx;
+      constexpr int kRBW = G::kRBW;
+      const int by = tid / (kRW / kRBW);
+      const int bx = tid - by * (kRW / kRBW);
+      const int rc0 = kRBW * bx;
       const int rr0 = 2 * by;
-      // window: sample rows rr0 .. rr0 + 1 + 2R, columns rc0 .. rc0 + 11
+      // window: sample rows rr0 .. rr0 + 1 + 2R, columns rc0 .. rc0 + kRBW + 3
       constexpr int kWR = 2 + 2 * kR;
-      float w[kWR][12];
+      constexpr int kWC = kRBW + 4;
+      float w[kWR][kWC];
 #pragma unroll
       for (int r = 0; r < kWR; ++r) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < kWC / 4; ++q) {
           const F4 a = *reinterpret_cast<const F4*>(&samp[(rr0 + r) * kSW + rc0 + 4 * q]);
 #pragma unroll
           for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
@@ -920,10 +965,10 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        float rgbs[8][3];
+        float rgbs[kRBW][3];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          // ring pixel parity: x = x0 - 2 + rc0 + i, y = y0 - 1 + rr0 + j
+        for (int i = 0; i < kRBW; ++i) {
+          // ring pixel parity: x = x0 - 2 + rc0 + i, y = y0 - 1 + rr0 + j (x0, rc0, y0, rr0 even)
           const int ch = cfa_channel(f.cfa, (((1 + j) & 1) << 1) | (i & 1));
           const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
           float first, second;
@@ -933,7 +978,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
 #pragma unroll
-          for (int q = 0; q < 2; ++q) {
+          for (int q = 0; q < kRBW / 4; ++q) {
             F4 o;
 #pragma unroll
             for (int i = 0; i < 4; ++i) o.v[i] = rgbs[4 * q + i][k];
@@ -1173,7 +1218,8 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
 }
 
 // Adds the block's sums and histograms to the frame's scratch; returns the
-// token of the atomics (see complete()).
+// token of the atomics (see complete_grid()).
+template <int kBlock = kThreads>
 __device__ __forceinline__ unsigned int main_flush(MainScratch* m, MainFrameLds& fl, const MainCtx& c,
                                                    const unsigned long long acc[4]) {
   const int tid = threadIdx.x;
@@ -1189,15 +1235,16 @@ __device__ __forceinline__ unsigned int main_flush(MainScratch* m, MainFrameLds&
   unsigned int token = 0;
   if (c.want_stats && tid < 4) {
     unsigned long long total = 0;
-    for (int w = 0; w < kThreads / 64; ++w) total += fl.part[w][tid];
-    if (total != 0) token ^= fold(atomicAdd(&m->sum[tid], total));
+    for (int w = 0; w < kBlock / 64; ++w) total += fl.part[w][tid];
+    if (total != 0) token ^= fold(atomicAdd(&m->sum[blockIdx.x % kSlots][tid], total));
   }
   if (c.want_hist) {
     const int bins = c.bins;
-    for (int i = tid; i < 4 * bins; i += kThreads) {
+    const int x = blockIdx.x % kSlots;
+    for (int i = tid; i < 4 * bins; i += kBlock) {
       unsigned int total = 0;
       for (int k = 0; k < c.copies; ++k) total += fl.hist[i * c.copies + k];
-      if (total != 0U) token ^= atomicAdd(&m->hist[i / bins][i % bins], total);
+      if (total != 0U) token ^= atomicAdd(&m->hist[x][i / bins][i % bins], total);
     }
   }
   return token;
@@ -1207,13 +1254,16 @@ __device__ __forceinline__ unsigned int main_flush(MainScratch* m, MainFrameLds&
 // (finish_stats, isp_math.hpp:387-425); the reads clear the scratch. With a
 // device exposure state, the frame runs at the state's exposure and leaves its
 // suggestion there for the stream's next frame.
+template <int kBlock = kThreads>
 __device__ __forceinline__ void main_finish(const FrameDesc& f, MainScratch* m, MainFrameLds& fl, const MainCtx& c) {
   const int tid = threadIdx.x;
   if (c.want_hist) {
     const int bins = c.bins;
-    for (int i = tid; i < 4 * bins; i += kThreads) f.histogram[i] = atomicExch(&m->hist[i / bins][i % bins], 0U);
+    for (int i = tid; i < 4 * bins; i += kBlock) {
+      f.histogram[i] = take_slots(&m->hist[0][i / bins][i % bins], 4 * kMaxHistBins);
+    }
   }
-  if (tid < 4) fl.sums[tid] = atomicExch(&m->sum[tid], 0ULL);
+  if (tid < 4) fl.sums[tid] = take_slots(&m->sum[0][tid], 4);
   __syncthreads();
   if (c.want_stats && tid == 0) {
     const unsigned long long sums[4] = {fl.sums[0], fl.sums[1], fl.sums[2], fl.sums[3]};
@@ -1235,14 +1285,16 @@ __device__ __forceinline__ void main_finish(const FrameDesc& f, MainScratch* m, 
       ae->valid = 1;
     }
   }
-  if (tid == 0) atomicExch(&m->done, 0U);
 }
 
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
-__global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles,
-                                                        int fast_rows, int out_aligned) {
+// kBlock threads per block: 512-thread blocks (one output row per thread) put
+// twice the waves on a CU for the same LDS, with registers held to what four
+// waves per SIMD leave.
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBlock >= 512 ? 4 : 1)))
+main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows, int out_aligned) {
   __shared__ MainFrameLds fl;
-  __shared__ MainTileLds<kFmt, kMalvar, kSharpen, kTileH> tl;
+  __shared__ MainTileLds<kFmt, kMalvar, kSharpen, kTileH, kBlock> tl;
   float gains[3];
   if (f.wb_mode != ISP_WB_MANUAL) {
 #pragma unroll
@@ -1252,14 +1304,15 @@ __global__ void __launch_bounds__(kThreads) main_kernel(FrameDesc f, DeviceScrat
     for (int k = 0; k < 3; ++k) gains[k] = f.wb[k];
   }
   MainCtx c;
-  main_begin(f, gains, fl, c);
+  main_begin<kBlock>(f, gains, fl, c);
   unsigned long long acc[4] = {0, 0, 0, 0};
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    main_tile<kFmt, kMalvar, kSharpen, kTileH>(f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
+    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock>(f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
   }
-  const unsigned int token = main_flush(&s->main, fl, c, acc);
-  if (!complete(&s->main.done, 1U, gridDim.x, token, &fl.flag)) return;
-  main_finish(f, &s->main, fl, c);
+  const unsigned int token = main_flush<kBlock>(&s->main, fl, c, acc);
+  if (!complete_grid(s->main.done, &s->main.top, token, &fl.flag)) return;
+  main_finish<kBlock>(f, &s->main, fl, c);
+  reset_grid(s->main.done, &s->main.top);
 }
 
 // ---- the batched form ----------------------------------------------------------
@@ -1348,7 +1401,7 @@ struct BatchQueue {
   int step = 0;  // static XCD: items between this block's turns, and its range's end
   int end = 0;
 
-  __device__ int first() {
+  __device__ __forceinline__ int first() {
     if (mode == ISP_BATCH_QUEUE_STATIC) return static_cast<int>(blockIdx.x);
     if (mode == ISP_BATCH_QUEUE_STATIC_XCD) {
       const int g = static_cast<int>(gridDim.x);
@@ -1367,10 +1420,10 @@ struct BatchQueue {
     return item;
   }
   // Dynamic: fetches the item after the current one (thread 0; read by next_item()).
-  __device__ void prefetch() {
+  __device__ __forceinline__ void prefetch() {
     if (mode == ISP_BATCH_QUEUE_DYNAMIC && threadIdx.x == 0) pending = gridDim.x + atomicAdd(next, 1U);
   }
-  __device__ int next_item(int item) {
+  __device__ __forceinline__ int next_item(int item) {
     if (mode == ISP_BATCH_QUEUE_STATIC) return item + static_cast<int>(gridDim.x);
     if (mode == ISP_BATCH_QUEUE_STATIC_XCD) return item + step < end ? item + step : total;
     if (threadIdx.x == 0) *slot = static_cast<int>(pending);
@@ -1379,7 +1432,7 @@ struct BatchQueue {
     __syncthreads();
     return next_one;
   }
-  __device__ void leave() {
+  __device__ __forceinline__ void leave() {
     if (mode == ISP_BATCH_QUEUE_DYNAMIC && threadIdx.x == 0) {
       const unsigned int ticket = atomicAdd(exited, 1U);
       if (ticket == gridDim.x - 1) {
@@ -1412,8 +1465,10 @@ __global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* _
       if (cur >= 0) {
         const BatchFrame& pb = table->frames[cur];
         const unsigned int token = awb_flush(&pb.s->awb, fl, ac, acc);
-        if (complete(&pb.s->awb.done, items_done, static_cast<unsigned int>(pb.awb_items), token, &fl.flag)) {
+        if (complete_items(&pb.s->awb.done[0], items_done, static_cast<unsigned int>(pb.awb_items), token,
+                           &fl.flag)) {
           awb_finish(pb.f, pb.s, fl, ac);
+          if (threadIdx.x == 0) atomicExch(&pb.s->awb.done[0], 0U);
         }
         __syncthreads();  // the epilogue is done with the frame LDS
 #pragma unroll
@@ -1442,8 +1497,9 @@ __global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* _
   if (cur >= 0) {
     const BatchFrame& pb = table->frames[cur];
     const unsigned int token = awb_flush(&pb.s->awb, fl, ac, acc);
-    if (complete(&pb.s->awb.done, items_done, static_cast<unsigned int>(pb.awb_items), token, &fl.flag)) {
+    if (complete_items(&pb.s->awb.done[0], items_done, static_cast<unsigned int>(pb.awb_items), token, &fl.flag)) {
       awb_finish(pb.f, pb.s, fl, ac);
+      if (threadIdx.x == 0) atomicExch(&pb.s->awb.done[0], 0U);
     }
   }
   q.leave();
@@ -1480,8 +1536,9 @@ __device__ __forceinline__ void batch_main_body(const BatchTable* __restrict__ t
   auto close = [&]() {
     const BatchFrame& pb = table->frames[cur];
     const unsigned int token = main_flush(&pb.s->main, fl, mc, acc);
-    if (complete(&pb.s->main.done, items_done, static_cast<unsigned int>(pb.items), token, &fl.flag)) {
+    if (complete_items(&pb.s->main.done[0], items_done, static_cast<unsigned int>(pb.items), token, &fl.flag)) {
       main_finish(pb.f, &pb.s->main, fl, mc);
+      if (threadIdx.x == 0) atomicExch(&pb.s->main.done[0], 0U);
     }
   };
   for (int item = q.first(); item < q.total; item = q.next_item(item)) {
@@ -1542,14 +1599,14 @@ void check(hipError_t status, const char* what) {
 }
 
 template <typename Kernel>
-int blocks_per_cu(Kernel kernel) {
+int blocks_per_cu(Kernel kernel, int threads = kThreads) {
   static std::mutex mutex;
   static std::map<const void*, int> cache;
   std::lock_guard<std::mutex> lock(mutex);
   const auto it = cache.find(reinterpret_cast<const void*>(kernel));
   if (it != cache.end()) return it->second;
   int blocks = 0;
-  check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kThreads, 0),
+  check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, threads, 0),
         "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   blocks = std::max(blocks, 1);
   cache[reinterpret_cast<const void*>(kernel)] = blocks;
@@ -1569,14 +1626,20 @@ bool out_aligned(const FrameDesc& f) {
 
 constexpr int kMainTileH = 32;
 
+// Threads per block of the per-frame main kernel: 512 with sharpening (whose LDS
+// allows two blocks per CU), 256 without (three blocks per CU).
+template <bool kSharpen>
+constexpr int kMainBlock = kSharpen ? 512 : 256;
+
 template <int kFmt, bool kMalvar, bool kSharpen>
 void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream) {
-  auto kernel = main_kernel<kFmt, kMalvar, kSharpen, kMainTileH>;
+  constexpr int kBlock = kMainBlock<kSharpen>;
+  auto kernel = main_kernel<kFmt, kMalvar, kSharpen, kMainTileH, kBlock>;
   const int tiles_x = (f.width + 63) / 64;
   const int tiles_y = (f.height + kMainTileH - 1) / kMainTileH;
   const int tiles = tiles_x * tiles_y;
-  const int grid = std::max(1, std::min(tiles, cfg.cu_count * blocks_per_cu(kernel)));
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, f, s, tiles_x, tiles, rows_aligned(f) ? 1 : 0,
+  const int grid = std::max(1, std::min(tiles, cfg.cu_count * blocks_per_cu(kernel, kBlock)));
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, stream, f, s, tiles_x, tiles, rows_aligned(f) ? 1 : 0,
                      out_aligned(f) ? 1 : 0);
   check(hipGetLastError(), "main_kernel launch");
 }

@@ -24,3 +24,16 @@ def test_no_contraction_in_kernels():
                 window = "\n".join(lines[max(0, i - 12):i + 12])
                 assert "v_div_scale_f32" in window and "v_div_fixup_f32" in window, \
                     f"{os.path.basename(path)}:{i + 1}: fused op outside a division: {line.strip()}"
+
+
+def test_kernels_make_no_calls_and_use_no_scratch():
+    """Every kernel is one function (no calls: a call spills the caller's state to
+    private memory) with no private segment (no register spills)."""
+    subprocess.run(["make", "-s", "-C", PKG, "asm"], check=True)
+    text = open(os.path.join(PKG, "_build", "isp_kernels.s")).read()
+    assert "s_swappc_b64" not in text  # (s_setpc_b64 alone is a long branch inside a large kernel)
+    kernels = re.findall(r"\.amdhsa_kernel (\S+)\n(.*?)\.end_amdhsa_kernel", text, re.S)
+    assert len(kernels) >= 20
+    for name, body in kernels:
+        scratch = int(re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", body).group(1))
+        assert scratch == 0, f"{name}: {scratch} bytes of scratch"
