@@ -93,6 +93,46 @@ ISP_HD dd dd_div(dd a, dd b) {
   return dd_add_d(q, q3);
 }
 
+// 1/j for j = 1..29 in double-double: hi the double nearest 1/j (what 1.0 / j
+// rounds to), lo the double nearest 1/j - hi.
+struct dd_bits {
+  std::uint64_t hi;
+  std::uint64_t lo;
+};
+inline constexpr dd_bits kRecip[30] = {
+    {0, 0},
+    {0x3ff0000000000000ULL, 0x0000000000000000ULL},  // 1/1
+    {0x3fe0000000000000ULL, 0x0000000000000000ULL},  // 1/2
+    {0x3fd5555555555555ULL, 0x3c75555555555555ULL},  // 1/3
+    {0x3fd0000000000000ULL, 0x0000000000000000ULL},  // 1/4
+    {0x3fc999999999999aULL, 0xbc6999999999999aULL},  // 1/5
+    {0x3fc5555555555555ULL, 0x3c65555555555555ULL},  // 1/6
+    {0x3fc2492492492492ULL, 0x3c62492492492492ULL},  // 1/7
+    {0x3fc0000000000000ULL, 0x0000000000000000ULL},  // 1/8
+    {0x3fbc71c71c71c71cULL, 0x3c5c71c71c71c71cULL},  // 1/9
+    {0x3fb999999999999aULL, 0xbc5999999999999aULL},  // 1/10
+    {0x3fb745d1745d1746ULL, 0xbc4745d1745d1746ULL},  // 1/11
+    {0x3fb5555555555555ULL, 0x3c55555555555555ULL},  // 1/12
+    {0x3fb3b13b13b13b14ULL, 0xbc53b13b13b13b14ULL},  // 1/13
+    {0x3fb2492492492492ULL, 0x3c52492492492492ULL},  // 1/14
+    {0x3fb1111111111111ULL, 0x3c31111111111111ULL},  // 1/15
+    {0x3fb0000000000000ULL, 0x0000000000000000ULL},  // 1/16
+    {0x3fae1e1e1e1e1e1eULL, 0x3c2e1e1e1e1e1e1eULL},  // 1/17
+    {0x3fac71c71c71c71cULL, 0x3c4c71c71c71c71cULL},  // 1/18
+    {0x3faaf286bca1af28ULL, 0x3c4af286bca1af28ULL},  // 1/19
+    {0x3fa999999999999aULL, 0xbc4999999999999aULL},  // 1/20
+    {0x3fa8618618618618ULL, 0x3c48618618618618ULL},  // 1/21
+    {0x3fa745d1745d1746ULL, 0xbc3745d1745d1746ULL},  // 1/22
+    {0x3fa642c8590b2164ULL, 0x3c3642c8590b2164ULL},  // 1/23
+    {0x3fa5555555555555ULL, 0x3c45555555555555ULL},  // 1/24
+    {0x3fa47ae147ae147bULL, 0xbc2eb851eb851eb8ULL},  // 1/25
+    {0x3fa3b13b13b13b14ULL, 0xbc43b13b13b13b14ULL},  // 1/26
+    {0x3fa2f684bda12f68ULL, 0x3c42f684bda12f68ULL},  // 1/27
+    {0x3fa2492492492492ULL, 0x3c42492492492492ULL},  // 1/28
+    {0x3fa1a7b9611a7b96ULL, 0x3c21a7b9611a7b96ULL},  // 1/29
+};
+ISP_HD dd dd_recip(int j) { return {bits_to_double(kRecip[j].hi), bits_to_double(kRecip[j].lo)}; }
+
 // ln 2 and log2(e) to 106 bits.
 ISP_HD dd dd_ln2() { return {bits_to_double(0x3fe62e42fefa39efULL), bits_to_double(0x3c7abc9e3b39803fULL)}; }
 ISP_HD dd dd_log2e() { return {bits_to_double(0x3ff71547652b82feULL), bits_to_double(0x3c7777d0ffda0d24ULL)}; }
@@ -120,12 +160,10 @@ ISP_HD double isp_log2(double x) {
   const dd s = dd_div(num, den);
   const dd t = dd_mul(s, s);
   // 2 atanh(s) / s = 2 (1 + t/3 + t^2/5 + ...): tail t^4/9 .. in double, head in double-double.
-  double tail = 1.0 / 29.0;
-  for (int j = 27; j >= 9; j -= 2) tail = 1.0 / static_cast<double>(j) + t.hi * tail;
+  double tail = dd_recip(29).hi;
+  for (int j = 27; j >= 9; j -= 2) tail = dd_recip(j).hi + t.hi * tail;
   dd poly{tail, 0.0};
-  for (int j = 7; j >= 1; j -= 2) {
-    poly = dd_add(dd_div(dd{1.0, 0.0}, dd{static_cast<double>(j), 0.0}), dd_mul(t, poly));
-  }
+  for (int j = 7; j >= 1; j -= 2) poly = dd_add(dd_recip(j), dd_mul(t, poly));
   const dd ln_m = dd_mul_d(dd_mul(s, poly), 2.0);
   const dd l2 = dd_mul(ln_m, dd_log2e());
   const dd r = dd_add(dd{static_cast<double>(k), 0.0}, l2);
@@ -142,7 +180,7 @@ ISP_HD double isp_exp2(double y) {
   // Taylor series by Horner: exp(q) = 1 + q (1 + q/2 (1 + q/3 (1 + ... q/10))).
   dd e{1.0, 0.0};
   for (int j = 10; j >= 1; --j) {
-    e = dd_add_d(dd_mul(dd_div(q, dd{static_cast<double>(j), 0.0}), e), 1.0);
+    e = dd_add_d(dd_mul(dd_mul(q, dd_recip(j)), e), 1.0);
   }
   for (int i = 0; i < 8; ++i) e = dd_mul(e, e);
   // Scale by 2^n exactly (n in the normal range here).

@@ -178,6 +178,7 @@ struct LaunchConfig {
   int cu_count;
   int tile_w;
   int tile_h;
+  int awb_blocks = 3;  // white balance kernel: resident blocks per CU at most
 };
 void launch_awb(const FrameDesc& f, DeviceScratch* scratch, const LaunchConfig& cfg, void* stream);
 void launch_main(const FrameDesc& f, DeviceScratch* scratch, bool malvar, bool sharpen,

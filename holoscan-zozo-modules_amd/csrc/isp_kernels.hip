@@ -1626,10 +1626,10 @@ bool out_aligned(const FrameDesc& f) {
 
 constexpr int kMainTileH = 32;
 
-// Threads per block of the per-frame main kernel: 512 with sharpening (whose LDS
-// allows two blocks per CU), 256 without (three blocks per CU).
+// Threads per block of the per-frame main kernel: 512, two blocks per CU (16
+// waves) with sharpening and without.
 template <bool kSharpen>
-constexpr int kMainBlock = kSharpen ? 512 : 256;
+constexpr int kMainBlock = 512;
 
 template <int kFmt, bool kMalvar, bool kSharpen>
 void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream) {
@@ -1660,7 +1660,7 @@ void launch_awb_f(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg,
   const int tiles_x = (f.width + kAwbTile - 1) / kAwbTile;
   const int tiles_y = (f.height + kAwbTile - 1) / kAwbTile;
   const int tiles = tiles_x * tiles_y;
-  const int grid = std::max(1, std::min(tiles, cfg.cu_count * std::min(4, blocks_per_cu(kernel))));
+  const int grid = std::max(1, std::min(tiles, cfg.cu_count * std::min(cfg.awb_blocks, blocks_per_cu(kernel))));
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, stream, f, s, tiles_x, tiles, rows_aligned(f) ? 1 : 0);
   check(hipGetLastError(), "awb_kernel launch");
 }
