@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--large-ticks", type=int, default=3, help="timed batches of 8 x 8192x6144 frames (0: skip)")
     ap.add_argument("--served-seconds", type=float, default=2.0,
                     help="real-time length of the 32-camera serving trace (0: skip)")
+    ap.add_argument("--source-launches", type=int, default=50,
+                    help="timed launches of the device raw source and the frame checker (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times): the reference "
                          "point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -336,6 +338,52 @@ def served_report(seconds):
     return out
 
 
+def source_and_check(launches):
+    """The device raw source (isp_synthetic_render_device) and the GPU frame checker
+    (isp_check_frame_device) on the reference demo's chart camera: each kernel's average
+    launch time from HIP events over `launches` launches on one stream, at 4K RAW12 and 1080p
+    u16, with the checked frame's PSNR and mismatched bins."""
+    import torch
+    import holoscan_isp as isp
+    from holoscan_isp import _abi
+    import cases
+    stream = torch.cuda.current_stream()
+    out = {"workload": "demo chart camera (vignetting 0.4, per-channel black), Malvar, manual white balance, "
+                       "RGB8; render then check", "launches": launches}
+    for name, w, h, packing in (("3840x2160_raw12", 3840, 2160, "raw12"), ("1920x1080_u16", 1920, 1080, "none")):
+        cam = cases.demo_camera(width=w, height=h, packing=packing, frame=3)
+        params = cases.demo_params(cam)
+        pitch = isp.lib().isp_raw_row_bytes(cam.format, w)
+        raw = torch.empty(pitch * h, dtype=torch.uint8, device="cuda")
+        rgb = torch.empty(3 * w * h, dtype=torch.uint8, device="cuda")
+        hist = torch.empty(1024, dtype=torch.int32, device="cuda")
+        stats = torch.empty(12, dtype=torch.float32, device="cuda")
+        pipe = isp.IspPipeline(params)
+        checker = isp.FrameChecker(params)
+        view = _abi.RawView(raw.data_ptr(), w, h, pitch, cam.format, cam.bit_depth, cam.pattern)
+
+        def timed(body):
+            for _ in range(3):
+                body()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(launches):
+                body()
+            e1.record(stream)
+            e1.synchronize()
+            return e0.elapsed_time(e1) / launches * 1e3
+
+        render_us = timed(lambda: isp.render_synthetic_device(cam, raw, pitch, stream.cuda_stream))
+        pipe.process(view, rgb, 3 * w, _abi.OUT_RGB8, hist, stats, cases.DEMO_EXPOSURE, stream.cuda_stream)
+        check_us = timed(lambda: checker.check(cam, rgb, hist, stats, 0, stream.cuda_stream))
+        r = checker.result()
+        out[name] = {"render_us": round(render_us, 2), "render_gpix_s": round(w * h / render_us / 1e3, 2),
+                     "check_us": round(check_us, 2), "check_gpix_s": round(w * h / check_us / 1e3, 2),
+                     "psnr_db": round(isp.frame_psnr(r, w, h), 3), "histogram_mismatches": r.histogram_mismatches}
+        pipe.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -449,6 +497,8 @@ def main():
         result["large_frames_8"] = large_frames(8, args.large_ticks)
     if args.served_seconds > 0:
         result["served_32_cameras"] = served_report(args.served_seconds)
+    if args.source_launches > 0:
+        result["device_source_and_check"] = source_and_check(args.source_launches)
     if not args.no_cpu:
         base = cpu_baseline(frame, params, out_format, args.cpu_frames)
         result["cpu_baseline"] = base["single"]

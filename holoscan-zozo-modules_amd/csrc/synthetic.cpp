@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/isp_synthetic.h"
+#include "chart.hpp"
 #include "isp_internal.hpp"
 
 namespace isp_amd {
@@ -104,223 +105,16 @@ void pack_row(int format, int width, const int* samples, std::uint8_t* row) {
 
 // ---- the chart camera ----------------------------------------------------
 
-struct Chart {
-  int width, height, kind, format;
-  int cfa[4];
-  float black[4];
-  float white;
-  float wb[3], inv_wb[3];
-  float ccm[9], inv_ccm[9];
-  float vignetting, exposure;
-  float cx, cy, inv_cx2, inv_cy2;
-  int shift;
-};
-
-// The 24 colour chart patches, linear sRGB (the reference's kPatches, synthetic_raw.hpp:103-112).
-constexpr float kPatch[24][3] = {
-    {0.1714F, 0.0844F, 0.0578F}, {0.5395F, 0.3050F, 0.2232F}, {0.1221F, 0.1946F, 0.3372F},
-    {0.0953F, 0.1500F, 0.0561F}, {0.2346F, 0.2159F, 0.4397F}, {0.1356F, 0.5089F, 0.4020F},
-    {0.6724F, 0.2086F, 0.0252F}, {0.0802F, 0.1046F, 0.3813F}, {0.5333F, 0.1022F, 0.1248F},
-    {0.1119F, 0.0452F, 0.1500F}, {0.3372F, 0.5029F, 0.0513F}, {0.7454F, 0.3663F, 0.0273F},
-    {0.0395F, 0.0467F, 0.3050F}, {0.0612F, 0.2961F, 0.0666F}, {0.4287F, 0.0369F, 0.0452F},
-    {0.7991F, 0.5711F, 0.0137F}, {0.4969F, 0.0931F, 0.3005F}, {0.0024F, 0.2346F, 0.3564F},
-    {0.8963F, 0.8963F, 0.8879F}, {0.5776F, 0.5776F, 0.5776F}, {0.3515F, 0.3515F, 0.3515F},
-    {0.1946F, 0.1946F, 0.1912F}, {0.0908F, 0.0908F, 0.0908F}, {0.0343F, 0.0343F, 0.0343F},
-};
-
-Chart make_chart(const isp_synthetic_config& c) {
-  if (c.width < 16 || c.height < 16 || c.width > 16384 || c.height > 16384)
-    throw invalid_argument("synthetic frames must be 16x16 to 16384x16384");
-  if (c.format > ISP_RAW_RAW12 || c.pattern > ISP_GBRG) throw invalid_argument("synthetic format or pattern unknown");
-  const bool packed = c.format == ISP_RAW_RAW10 || c.format == ISP_RAW_RAW12;
-  if (c.bit_depth < 2 || c.bit_depth > format_bits(c.format) || (packed && c.bit_depth != format_bits(c.format)))
-    throw invalid_argument("synthetic bit depth does not fit its sample format");
-  if (!(c.vignetting >= 0.0F && c.vignetting <= 0.9F)) throw invalid_argument("synthetic vignetting must be 0..0.9");
-  if (!(std::isfinite(c.exposure) && c.exposure > 0.0F && c.exposure <= 64.0F))
-    throw invalid_argument("synthetic exposure must be in (0, 64]");
-  Chart s{};
-  s.width = c.width;
-  s.height = c.height;
-  s.kind = c.kind;
-  s.format = c.format;
-  for (int k = 0; k < 4; ++k) s.cfa[k] = kCfaTable[c.pattern][k];
-  s.white = static_cast<float>((1U << static_cast<unsigned>(c.bit_depth)) - 1U);
-  for (int k = 0; k < 4; ++k) {
-    if (!(c.black_levels[k] >= 0.0F && c.black_levels[k] < s.white / 2.0F))
-      throw invalid_argument("synthetic black levels must be below half the white level");
-    s.black[k] = c.black_levels[k];
-  }
-  for (int k = 0; k < 3; ++k) {
-    const float g = c.wb_gains[k];
-    if (!(std::isfinite(g) && g >= 0.25F && g <= 8.0F)) throw invalid_argument("synthetic wb_gains must be 0.25..8");
-    s.wb[k] = g;
-    s.inv_wb[k] = 1.0F / g;
-  }
-  // The matrix inverse by cofactors in double (synthetic_raw.cpp:21-35).
-  const double a = c.ccm[0], b = c.ccm[1], cc = c.ccm[2], d = c.ccm[3], e = c.ccm[4], f = c.ccm[5],
-               g = c.ccm[6], h = c.ccm[7], i = c.ccm[8];
-  const double det = a * (e * i - f * h) - b * (d * i - f * g) + cc * (d * h - e * g);
-  if (!(std::abs(det) > 1e-6)) throw invalid_argument("synthetic ccm must be invertible");
-  const double cof[9] = {(e * i - f * h),  -(b * i - cc * h), (b * f - cc * e),
-                         -(d * i - f * g), (a * i - cc * g),  -(a * f - cc * d),
-                         (d * h - e * g),  -(a * h - b * g),  (a * e - b * d)};
-  for (int k = 0; k < 9; ++k) {
-    s.ccm[k] = c.ccm[k];
-    s.inv_ccm[k] = static_cast<float>(cof[k] / det);
-  }
-  s.vignetting = c.vignetting;
-  s.exposure = c.exposure;
-  s.cx = static_cast<float>(c.width - 1) / 2.0F;
-  s.cy = static_cast<float>(c.height - 1) / 2.0F;
-  s.inv_cx2 = 1.0F / (s.cx * s.cx);
-  s.inv_cy2 = 1.0F / (s.cy * s.cy);
-  const unsigned span = static_cast<unsigned>(c.width / 32 > 1 ? c.width / 32 : 1);
-  const unsigned phase = c.frame % (2U * span);
-  s.shift = 2 * static_cast<int>(phase < span ? phase : 2U * span - phase);
-  return s;
-}
-
-void set3(float* rgb, float r, float g, float b) {
-  rgb[0] = r;
-  rgb[1] = g;
-  rgb[2] = b;
-}
-
-// The scene at point (fx, fy) (synthetic_scene_point, synthetic_raw.hpp:152-236).
-void scene_point(const Chart& s, float fx, float fy, float rgb[3]) {
-  const int w = s.width;
-  const int h = s.height;
-  if (s.kind == ISP_SCENE_GRAY) {
-    float v;
-    if (fx < static_cast<float>(w / 2)) {
-      v = 0.03F + 0.8F * fy / static_cast<float>(h - 1);
-    } else {
-      const float cell = static_cast<float>(h / 4 > 1 ? h / 4 : 1);
-      const int col = static_cast<int>(std::floor((fx - static_cast<float>(w / 2 - s.shift)) / cell)) & 3;
-      const int row = static_cast<int>(std::floor(fy / cell)) & 3;
-      v = 0.05F + 0.8F * static_cast<float>((row * 4 + col) * 7 % 16) / 15.0F;
-    }
-    set3(rgb, v, v, v);
-    return;
-  }
-  if (fy < static_cast<float>(h * 58 / 100)) {
-    // The colour chart: 6 x 4 patches.
-    const int bottom = h * 58 / 100;
-    const int left = w / 24;
-    const int top = h / 30;
-    const int cell_w = (w - 2 * left) / 6;
-    const int cell_h = (bottom - 2 * top) / 4;
-    const int patch_w = cell_w - cell_w / 6;
-    const int patch_h = cell_h - cell_h / 6;
-    set3(rgb, 0.05F, 0.05F, 0.05F);
-    const float px = fx - static_cast<float>(left + s.shift);
-    const float py = fy - static_cast<float>(top);
-    if (px >= 0.0F && py >= 0.0F) {
-      const int col = static_cast<int>(px) / cell_w;
-      const int row = static_cast<int>(py) / cell_h;
-      if (col < 6 && row < 4 && px - static_cast<float>(col * cell_w) < static_cast<float>(patch_w) &&
-          py - static_cast<float>(row * cell_h) < static_cast<float>(patch_h)) {
-        const float* p = kPatch[row * 6 + col];
-        set3(rgb, p[0], p[1], p[2]);
-      }
-    }
-  } else if (fy < static_cast<float>(h * 70 / 100)) {
-    const float v = 0.01F + 0.89F * fx / static_cast<float>(w - 1);
-    set3(rgb, v, v, v);
-  } else if (fy < static_cast<float>(h * 82 / 100)) {
-    float hue = 6.0F * fx / static_cast<float>(w);
-    hue = hue < 0.0F ? 0.0F : hue;
-    const int sector = static_cast<int>(hue) % 6;
-    const float frac = hue - static_cast<float>(static_cast<int>(hue));
-    const float lo = 0.05F;
-    const float hi = 0.7F;
-    const float up = lo + (hi - lo) * frac;
-    const float down = hi - (hi - lo) * frac;
-    switch (sector) {
-      case 0: set3(rgb, hi, up, lo); break;
-      case 1: set3(rgb, down, hi, lo); break;
-      case 2: set3(rgb, lo, hi, up); break;
-      case 3: set3(rgb, lo, down, hi); break;
-      case 4: set3(rgb, up, lo, hi); break;
-      default: set3(rgb, hi, lo, down); break;
-    }
-  } else if (fx < static_cast<float>(w / 2)) {
-    const float third = static_cast<float>(w / 2 > 2 ? w / 2 : 2) / 3.0F;
-    const int region = fx < third ? 0 : fx < 2.0F * third ? 1 : 2;
-    const float bar = static_cast<float>(4 << region);
-    const float v = (static_cast<int>(std::floor(fx / bar)) & 1) != 0 ? 0.85F : 0.04F;
-    set3(rgb, v, v, v);
-  } else {
-    const int top = h * 82 / 100;
-    const float dx = fx - static_cast<float>(3 * w / 4 + s.shift);
-    const float dy = fy - static_cast<float>((top + h) / 2);
-    const float radius = static_cast<float>((h - top) / 3);
-    if (dx * dx + dy * dy < radius * radius) {
-      set3(rgb, 0.6F, 0.08F, 0.06F);
-    } else {
-      const float v = 5.0F * dx + dy > 0.0F ? 0.75F : 0.12F;
-      set3(rgb, v, v, v);
-    }
-  }
-}
-
-// The lens-blurred scene at pixel (x, y): the mean of 4 x 4 points (synthetic_raw.hpp:241-253).
-void scene_rgb(const Chart& s, int x, int y, float rgb[3]) {
-  float sum[3] = {0.0F, 0.0F, 0.0F};
-  for (int j = 0; j < 4; ++j) {
-    for (int i = 0; i < 4; ++i) {
-      float p[3];
-      scene_point(s, static_cast<float>(x) + 0.5F * static_cast<float>(i) - 0.75F,
-                  static_cast<float>(y) + 0.5F * static_cast<float>(j) - 0.75F, p);
-      for (int k = 0; k < 3; ++k) sum[k] = sum[k] + p[k];
-    }
-  }
-  for (int k = 0; k < 3; ++k) rgb[k] = sum[k] * 0.0625F * s.exposure;
-}
-
-float falloff(const Chart& s, int x, int y) {
-  const float dx = static_cast<float>(x) - s.cx;
-  const float dy = static_cast<float>(y) - s.cy;
-  return 1.0F - s.vignetting * ((dx * dx * s.inv_cx2 + dy * dy * s.inv_cy2) * 0.5F);
-}
-
-void camera_rgb(const Chart& s, int x, int y, float cam[3]) {
-  float rgb[3];
-  scene_rgb(s, x, y, rgb);
-  for (int k = 0; k < 3; ++k) {
-    const float v = s.inv_ccm[3 * k] * rgb[0] + s.inv_ccm[3 * k + 1] * rgb[1] + s.inv_ccm[3 * k + 2] * rgb[2];
-    cam[k] = v > 0.0F ? v : 0.0F;
-  }
-}
-
 int chart_sample(const Chart& s, int x, int y) {
-  float cam[3];
-  camera_rgb(s, x, y, cam);
-  const int channel = s.cfa[((y & 1) << 1) | (x & 1)];
-  const int colour = colour_of_channel(channel);
-  float signal = cam[colour] * falloff(s, x, y) * s.inv_wb[colour];
-  signal = signal < 1.0F ? signal : 1.0F;
-  const float counts = s.black[channel] + signal * (s.white - s.black[channel]);
-  const int value = static_cast<int>(counts + 0.5F);
-  const int white = static_cast<int>(s.white);
-  return value < 0 ? 0 : value > white ? white : value;
+  float rgb[3];
+  chart_scene(s, x, y, rgb);
+  return chart_sample_of(s, rgb, x, y);
 }
 
 void chart_truth(const Chart& s, int x, int y, float linear[3]) {
-  float cam[3];
-  camera_rgb(s, x, y, cam);
-  const float fall = falloff(s, x, y);
-  float balanced[3];
-  for (int k = 0; k < 3; ++k) {
-    float signal = cam[k] * fall * s.inv_wb[k];
-    signal = signal < 1.0F ? signal : 1.0F;
-    const float v = signal / fall * s.wb[k];
-    balanced[k] = v < 1.0F ? v : 1.0F;
-  }
-  for (int k = 0; k < 3; ++k) {
-    const float v = s.ccm[3 * k] * balanced[0] + s.ccm[3 * k + 1] * balanced[1] + s.ccm[3 * k + 2] * balanced[2];
-    linear[k] = v < 0.0F ? 0.0F : v > 1.0F ? 1.0F : v;
-  }
+  float rgb[3];
+  chart_scene(s, x, y, rgb);
+  chart_truth_of(s, rgb, x, y, linear);
 }
 
 // ---- the noisy scene ------------------------------------------------------
@@ -454,6 +248,59 @@ isp_status guarded(F&& body) {
 }
 
 }  // namespace
+
+Chart make_chart(const isp_synthetic_config& c) {
+  if (c.width < 16 || c.height < 16 || c.width > 16384 || c.height > 16384)
+    throw invalid_argument("synthetic frames must be 16x16 to 16384x16384");
+  if (c.format > ISP_RAW_RAW12 || c.pattern > ISP_GBRG) throw invalid_argument("synthetic format or pattern unknown");
+  const bool packed = c.format == ISP_RAW_RAW10 || c.format == ISP_RAW_RAW12;
+  if (c.bit_depth < 2 || c.bit_depth > format_bits(c.format) || (packed && c.bit_depth != format_bits(c.format)))
+    throw invalid_argument("synthetic bit depth does not fit its sample format");
+  if (!(c.vignetting >= 0.0F && c.vignetting <= 0.9F)) throw invalid_argument("synthetic vignetting must be 0..0.9");
+  if (!(std::isfinite(c.exposure) && c.exposure > 0.0F && c.exposure <= 64.0F))
+    throw invalid_argument("synthetic exposure must be in (0, 64]");
+  Chart s{};
+  s.width = c.width;
+  s.height = c.height;
+  s.kind = c.kind;
+  s.format = c.format;
+  for (int k = 0; k < 4; ++k) s.cfa[k] = kCfaTable[c.pattern][k];
+  s.white = static_cast<float>((1U << static_cast<unsigned>(c.bit_depth)) - 1U);
+  for (int k = 0; k < 4; ++k) {
+    if (!(c.black_levels[k] >= 0.0F && c.black_levels[k] < s.white / 2.0F))
+      throw invalid_argument("synthetic black levels must be below half the white level");
+    s.black[k] = c.black_levels[k];
+  }
+  for (int k = 0; k < 3; ++k) {
+    const float g = c.wb_gains[k];
+    if (!(std::isfinite(g) && g >= 0.25F && g <= 8.0F)) throw invalid_argument("synthetic wb_gains must be 0.25..8");
+    s.wb[k] = g;
+    s.inv_wb[k] = 1.0F / g;
+  }
+  // The matrix inverse by cofactors in double (synthetic_raw.cpp:21-35).
+  const double a = c.ccm[0], b = c.ccm[1], cc = c.ccm[2], d = c.ccm[3], e = c.ccm[4], f = c.ccm[5],
+               g = c.ccm[6], h = c.ccm[7], i = c.ccm[8];
+  const double det = a * (e * i - f * h) - b * (d * i - f * g) + cc * (d * h - e * g);
+  if (!(std::abs(det) > 1e-6)) throw invalid_argument("synthetic ccm must be invertible");
+  const double cof[9] = {(e * i - f * h),  -(b * i - cc * h), (b * f - cc * e),
+                         -(d * i - f * g), (a * i - cc * g),  -(a * f - cc * d),
+                         (d * h - e * g),  -(a * h - b * g),  (a * e - b * d)};
+  for (int k = 0; k < 9; ++k) {
+    s.ccm[k] = c.ccm[k];
+    s.inv_ccm[k] = static_cast<float>(cof[k] / det);
+  }
+  s.vignetting = c.vignetting;
+  s.exposure = c.exposure;
+  s.cx = static_cast<float>(c.width - 1) / 2.0F;
+  s.cy = static_cast<float>(c.height - 1) / 2.0F;
+  s.inv_cx2 = 1.0F / (s.cx * s.cx);
+  s.inv_cy2 = 1.0F / (s.cy * s.cy);
+  const unsigned span = static_cast<unsigned>(c.width / 32 > 1 ? c.width / 32 : 1);
+  const unsigned phase = c.frame % (2U * span);
+  s.shift = 2 * static_cast<int>(phase < span ? phase : 2U * span - phase);
+  return s;
+}
+
 }  // namespace isp_amd
 
 using namespace isp_amd;

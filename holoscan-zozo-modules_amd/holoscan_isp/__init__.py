@@ -378,3 +378,41 @@ class IspServer:
 
 def now_ns():
     return lib().isp_server_now_ns()
+
+
+def render_synthetic_device(config, data, pitch=None, stream=None):
+    """render_synthetic_raw_cuda (synthetic_raw_cuda.hpp:17-19): the frame of `config` into
+    device memory `data` (a tensor or address), rows `pitch` bytes apart (None: the packed row
+    rounded up to even bytes, as render_synthetic), queued on `stream`. Returns the pitch."""
+    row = lib().isp_raw_row_bytes(config.format, config.width)
+    if pitch is None:
+        pitch = (row + 1) & ~1
+    check(lib().isp_synthetic_render_device(C.byref(config), _ptr(data), pitch, stream), "render_synthetic_device")
+    return pitch
+
+
+class FrameChecker:
+    """The demo checker's device state (check_op.cpp:104-140) around check_frame_cuda
+    (frame_check.hpp:70-76): the tone table of `params`, the zeroed scratch and the 64-byte
+    result in device memory. check() queues one frame's check; result() reads it back."""
+
+    def __init__(self, params):
+        import torch
+        self.lut = torch.from_numpy(np.asarray(tone_lut(params), dtype=np.float32)).cuda()
+        self.scratch = torch.zeros(FRAME_CHECK_SCRATCH_BYTES, dtype=torch.uint8, device="cuda")
+        self.device_result = torch.zeros(C.sizeof(_abi.FrameCheckResult), dtype=torch.uint8, device="cuda")
+
+    def check(self, camera, rgb, histogram, stats=None, pitch=0, stream=None):
+        check(lib().isp_check_frame_device(C.byref(camera), _ptr(self.lut), int(self.lut.numel()), _ptr(rgb), pitch,
+                                           _ptr(histogram), _ptr(stats), _ptr(self.scratch),
+                                           _ptr(self.device_result), stream), "check_frame_device")
+
+    def result(self):
+        """The last check's isp_frame_check_result (waits for the device)."""
+        host = self.device_result.cpu().numpy()
+        return _abi.FrameCheckResult.from_buffer_copy(host.tobytes())
+
+
+def frame_psnr(result, width, height):
+    """PSNR in dB of a checked frame (check_op.cpp:226-228)."""
+    return lib().isp_frame_check_psnr(C.byref(result), width, height)

@@ -1,4 +1,4 @@
-"""ctypes mirror of include/isp_c.h and include/isp_synthetic.h.
+"""ctypes mirror of include/isp_c.h, include/isp_synthetic.h, include/isp_source.h and include/isp_op.h.
 
 The field order and types follow the C headers exactly; the names follow the
 reference's holoscan::isp types (modules/isp/include/holoscan/isp/isp_params.hpp).
@@ -179,6 +179,13 @@ class SceneConfig(C.Structure):
                 ("read_sigma", C.c_float), ("illumination", C.c_float * 3)]
 
 
+class FrameCheckResult(C.Structure):
+    _fields_ = [("squared_error", C.c_uint64), ("histogram_mismatches", C.c_uint32),
+                ("reserved", C.c_uint32), ("stats", C.c_float * STATS_SIZE)]
+
+
+FRAME_CHECK_SCRATCH_BYTES = 32896
+
 P = C.POINTER
 _void = C.c_void_p
 
@@ -253,6 +260,10 @@ PROTOTYPES = [
     ("isp_scene_config_init", None, [P(SceneConfig)]),
     ("isp_scene_render", C.c_int, [P(SceneConfig), _void, C.c_size_t]),
     ("isp_scene_lens_shading", C.c_int, [P(SceneConfig), C.c_int32, C.c_int32, P(C.c_float)]),
+    ("isp_synthetic_render_device", C.c_int, [P(SyntheticConfig), _void, C.c_size_t, _void]),
+    ("isp_check_frame_device", C.c_int, [P(SyntheticConfig), _void, C.c_int32, _void, C.c_size_t, _void,
+                                         _void, _void, _void, _void]),
+    ("isp_frame_check_psnr", C.c_double, [P(FrameCheckResult), C.c_int32, C.c_int32]),
 ]
 
 

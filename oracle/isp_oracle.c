@@ -24,6 +24,11 @@
  *    gain 1.5), histogram rebuild from the image, black frame.
  *  - the 1024-bin histogram extension (no reference counterpart) is pinned by
  *    rebuilding it from the written RGB16 / RGB32F image only.
+ *  - the frame checker (oracle_frame_check) by the reference demo's published
+ *    run (applications/isp/README.md:89: mean and worst PSNR, mean luma and AE
+ *    error of 30 checked 640x480 frames) and by closed forms: a perfect image
+ *    scores 0 error and 99 dB, a pixel off by d adds d^2, a changed bin counts
+ *    one mismatch, and the rebuilt rows equal the ISP's histograms.
  */
 #include "isp_oracle.h"
 
@@ -716,4 +721,48 @@ int oracle_process(oracle_pipeline* o, const isp_raw_view* raw, const isp_rgb_vi
   }
   free(jobs);
   return 0;
+}
+
+/* ---- the demo's frame checker ------------------------------------------- */
+
+/* tone (applications/isp/src/frame_check.hpp:45-51): the table at v in 0..1,
+ * interpolated linearly. */
+static float check_tone(const float* lut, int size, float v) {
+  const float f = v * (float)(size - 1);
+  const int i = (int)f;
+  return i >= size - 1 ? lut[size - 1] : lut[i] + (f - (float)i) * (lut[i + 1] - lut[i]);
+}
+
+/* check_frame_host (check_op.cpp:44-72), with truth_codes (frame_check.hpp:54-62)
+ * and luma_bin (frame_check.hpp:64-67). */
+void oracle_frame_check(const float* truth, int width, int height, const float* lut, int lut_size,
+                        const uint8_t* rgb, size_t pitch, const uint32_t* histogram,
+                        const float* stats, isp_frame_check_result* result) {
+  uint32_t rebuilt[ISP_HISTOGRAM_CHANNELS * ISP_HISTOGRAM_BINS];
+  memset(rebuilt, 0, sizeof(rebuilt));
+  memset(result, 0, sizeof(*result));
+  for (int y = 0; y < height; ++y) {
+    for (int x = 0; x < width; ++x) {
+      const float* t = truth + ((size_t)y * width + x) * 3;
+      const uint8_t* px = rgb + (size_t)y * pitch + (size_t)x * 3;
+      for (int k = 0; k < 3; ++k) {
+        const int code = (int)round(255.0 * (double)check_tone(lut, lut_size, t[k]));
+        const int d = (int)px[k] - code;
+        result->squared_error += (uint64_t)(d * d);
+      }
+      const int q[3] = {px[0], px[1], px[2]};
+      ++rebuilt[0 * ISP_HISTOGRAM_BINS + luma_code(q)];
+      ++rebuilt[1 * ISP_HISTOGRAM_BINS + q[0]];
+      ++rebuilt[2 * ISP_HISTOGRAM_BINS + q[1]];
+      ++rebuilt[3 * ISP_HISTOGRAM_BINS + q[2]];
+    }
+  }
+  for (int i = 0; i < ISP_HISTOGRAM_CHANNELS * ISP_HISTOGRAM_BINS; ++i)
+    result->histogram_mismatches += rebuilt[i] != histogram[i] ? 1U : 0U;
+  for (int k = 0; k < ISP_STATS_SIZE; ++k) result->stats[k] = stats != NULL ? stats[k] : 0.0F;
+}
+
+double oracle_frame_psnr(const isp_frame_check_result* result, int width, int height) {
+  const double squared = (double)result->squared_error;
+  return squared == 0.0 ? 99.0 : 10.0 * log10(255.0 * 255.0 * 3.0 * (double)width * (double)height / squared);
 }

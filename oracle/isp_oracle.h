@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include "../include/isp_c.h"
+#include "../include/isp_source.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -46,6 +47,17 @@ void oracle_finish_stats(const uint64_t sums[4], int64_t pixels, const float gai
                          float* out);
 /* histogram_percentile (isp_math.hpp:336-352) over a 1024-bin histogram. */
 float oracle_histogram_percentile(const uint32_t* hist, float percentile);
+
+/* The frame checker of the reference's demo on the host (check_frame_host,
+ * applications/isp/src/check_op.cpp:44-72): `truth` holds the camera's linear
+ * truth, width*height*3 floats (synthetic_truth per pixel); `rgb` the RGB8
+ * image, rows `pitch` bytes apart; `histogram` the 4 x 256 rows to compare;
+ * `stats` (may be NULL: zeros) is copied into the result. */
+void oracle_frame_check(const float* truth, int width, int height, const float* lut, int lut_size,
+                        const uint8_t* rgb, size_t pitch, const uint32_t* histogram,
+                        const float* stats, isp_frame_check_result* result);
+/* The PSNR of a checked frame (check_op.cpp:226-228). */
+double oracle_frame_psnr(const isp_frame_check_result* result, int width, int height);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,8 @@
 """Shared test cases: synthetic-camera frames and settings, as the reference's
 GPU suite builds them (modules/isp/tests/pipeline_test.cpp:45-88, :164-235),
 plus the seeded noisy scenes of the benchmark configurations."""
+import numpy as np
+
 import holoscan_isp as isp
 from holoscan_isp import _abi
 
@@ -210,3 +212,46 @@ def poisson_arrivals(cameras, fps, duration, seed=401):
             t += rng.expovariate(fps)
     events.sort()
     return events
+
+
+# ---- the reference demo's camera (applications/isp/src/main.cpp) -------------
+
+def demo_camera(width=640, height=480, pattern=_abi.RGGB, bits=12, packing="none", scene=_abi.SCENE_CHART,
+                frame=0):
+    """camera_of (main.cpp:214-228) after the option rules of main.cpp:177-180: packed RAW10/RAW12
+    force 10/12 bits, 8 bits or fewer travel as uint8; black about 6 % of full scale, a little
+    higher on the second and fourth channel; vignetting 0.4."""
+    import math
+    fmt = {"raw10": _abi.RAW_RAW10, "raw12": _abi.RAW_RAW12}.get(packing, _abi.RAW_U16)
+    if fmt == _abi.RAW_RAW10:
+        bits = 10
+    if fmt == _abi.RAW_RAW12:
+        bits = 12
+    if fmt == _abi.RAW_U16 and bits <= 8:
+        fmt = _abi.RAW_U8
+    black = float(math.floor(float(np.float32(0.0625) * np.float32((1 << bits) - 1)) + 0.5))  # std::round, >= 0
+    step = float(math.floor(black / 32.0))
+    return isp.synthetic_config(width=width, height=height, pattern=pattern, format=fmt, bit_depth=bits,
+                                black_levels=(black, black + step, black, black + 2.0 * step), vignetting=0.4,
+                                kind=scene, frame=frame)
+
+
+def demo_params(camera, demosaic=_abi.MALVAR, white_balance=_abi.WB_MANUAL, sharpen=0.0):
+    """The demo's IspOp params (main.cpp:254-261): the camera's settings, the chosen demosaic,
+    white balance and sharpening; an estimating white balance starts from neutral gains."""
+    p = isp.synthetic_params(camera, 17)
+    p.demosaic = demosaic
+    p.white_balance = white_balance
+    p.sharpen = sharpen
+    if white_balance != _abi.WB_MANUAL:
+        p.wb_gains = (1.0, 1.0, 1.0)
+    return p
+
+
+def demo_min_psnr(bits, demosaic):
+    """The bound every frame must beat (main.cpp:397-400)."""
+    return 24.0 if bits <= 8 else 28.0 if demosaic == _abi.MALVAR else 26.0
+
+
+# The demo's exposure descriptor (raw_source_op.hpp:28-29).
+DEMO_EXPOSURE = (10_000_000, 1.0)

@@ -29,6 +29,9 @@ ORACLE_PROTOTYPES = [
     ("oracle_finish_stats", None, [P(C.c_uint64), C.c_int64, P(C.c_float), P(_abi.AutoExposure),
                                    P(_abi.FrameExposure), P(C.c_float)]),
     ("oracle_histogram_percentile", C.c_float, [P(C.c_uint32), C.c_float]),
+    ("oracle_frame_check", None, [P(C.c_float), C.c_int, C.c_int, P(C.c_float), C.c_int, _v, C.c_size_t,
+                                  P(C.c_uint32), P(C.c_float), P(_abi.FrameCheckResult)]),
+    ("oracle_frame_psnr", C.c_double, [P(_abi.FrameCheckResult), C.c_int, C.c_int]),
 ]
 
 REF_PROTOTYPES = [
@@ -161,3 +164,24 @@ class RefPipeline:
         if rc != 0:
             raise ValueError(ref().ref_last_error().decode())
         return Result(image, hist, stats)
+
+
+def frame_check(truth, lut, rgb, histogram, stats=None):
+    """The oracle's frame check (check_frame_host): truth float32 [H, W, 3], lut float32, rgb
+    uint8 [H, W, 3] (packed rows), histogram uint32 [4 * 256], stats float32 [12] or None."""
+    h, w = truth.shape[:2]
+    truth = np.ascontiguousarray(truth, dtype=np.float32)
+    lut = np.ascontiguousarray(lut, dtype=np.float32)
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    histogram = np.ascontiguousarray(histogram, dtype=np.uint32)
+    fp = C.POINTER(C.c_float)
+    st = None if stats is None else np.ascontiguousarray(stats, dtype=np.float32)
+    r = _abi.FrameCheckResult()
+    oracle().oracle_frame_check(truth.ctypes.data_as(fp), w, h, lut.ctypes.data_as(fp), lut.size,
+                                rgb.ctypes.data, 3 * w, histogram.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                None if st is None else st.ctypes.data_as(fp), C.byref(r))
+    return r
+
+
+def frame_psnr(result, width, height):
+    return oracle().oracle_frame_psnr(C.byref(result), width, height)
