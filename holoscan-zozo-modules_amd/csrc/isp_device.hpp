@@ -44,7 +44,7 @@ __device__ __forceinline__ bool complete_items(unsigned int* done, unsigned int 
 // so that no counter takes more than a kSlots-th of the blocks: block b counts
 // on done[b % kSlots]; the block that completes its slot's group (whose size
 // the grid fixes) counts on `top`; the one that completes `top` is the last.
-__device__ __forceinline__ bool complete_grid(unsigned int* done, unsigned int* top, unsigned int token, int* flag) {
+__device__ __forceinline__ bool complete_grid(LineCounter* done, LineCounter* top, unsigned int token, int* flag) {
   __asm__ volatile("" : : "v"(token));
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -53,7 +53,7 @@ __device__ __forceinline__ bool complete_grid(unsigned int* done, unsigned int* 
     const unsigned int groups = g < kSlots ? g : kSlots;
     const unsigned int members = (g - x + kSlots - 1) / kSlots;
     int last = 0;
-    if (atomicAdd(&done[x], 1U) + 1U == members) last = atomicAdd(top, 1U) + 1U == groups ? 1 : 0;
+    if (atomicAdd(&done[x].n, 1U) + 1U == members) last = atomicAdd(&top->n, 1U) + 1U == groups ? 1 : 0;
     *flag = last;
   }
   __syncthreads();
@@ -61,9 +61,9 @@ __device__ __forceinline__ bool complete_grid(unsigned int* done, unsigned int* 
 }
 
 // The last block's reset of complete_grid's counters (every block has counted).
-__device__ __forceinline__ void reset_grid(unsigned int* done, unsigned int* top) {
-  if (threadIdx.x < kSlots) atomicExch(&done[threadIdx.x], 0U);
-  if (threadIdx.x == 0) atomicExch(top, 0U);
+__device__ __forceinline__ void reset_grid(LineCounter* done, LineCounter* top) {
+  if (threadIdx.x < kSlots) atomicExch(&done[threadIdx.x].n, 0U);
+  if (threadIdx.x == 0) atomicExch(&top->n, 0U);
 }
 
 // Reads and clears the kSlots copies of an accumulator, returning their sum.

@@ -98,26 +98,34 @@ struct FrameDesc {
 // what it consumed). The blocks of a launch add into kSlots copies of every
 // accumulator, block b into copy b % kSlots, so that no address takes more than
 // a kSlots-th of the blocks' device-scope atomics (same-address atomics
-// serialise where they are performed); the last block adds the copies up.
+// serialise where they are performed); the last block adds the copies up. The
+// sum copies and the completion counters each sit alone on a 256-byte line:
+// atomics to one line queue behind each other even at different addresses.
 inline constexpr int kSlots = 8;
 
+struct alignas(256) LineCounter {
+  unsigned int n;
+};
+struct alignas(256) LineSums {
+  unsigned long long v[4];
+};
+inline constexpr std::size_t kLineSumsStride = sizeof(LineSums) / sizeof(unsigned long long);
+
 struct AwbScratch {
-  unsigned long long sum[kSlots][4];  // R, G, B fixed sums; complete quads
+  LineSums sum[kSlots];  // R, G, B fixed sums; complete quads
   unsigned int hist[kSlots][3][kAwbBins];
-  unsigned int done[kSlots];  // blocks finished, per slot
-  unsigned int top;           // slots finished
-  unsigned int pad[7];
+  LineCounter done[kSlots];  // blocks finished, per slot
+  LineCounter top;           // slots finished
 };
 struct WbState {
   float gains[3];
   int valid;
 };
 struct MainScratch {
-  unsigned long long sum[kSlots][4];  // R, G, B fixed sums; saturated samples
+  LineSums sum[kSlots];  // R, G, B fixed sums; saturated samples
   unsigned int hist[kSlots][4][kMaxHistBins];
-  unsigned int done[kSlots];
-  unsigned int top;
-  unsigned int pad[7];
+  LineCounter done[kSlots];
+  LineCounter top;
 };
 struct DeviceScratch {
   AwbScratch awb;

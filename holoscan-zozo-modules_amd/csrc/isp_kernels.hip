@@ -490,7 +490,7 @@ __device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl
   if (tid < 4) {
     unsigned long long total = 0;
     for (int w = 0; w < kThreads / 64; ++w) total += fl.part[w][tid];
-    if (total != 0) token ^= fold(atomicAdd(&a->sum[blockIdx.x % kSlots][tid], total));
+    if (total != 0) token ^= fold(atomicAdd(&a->sum[blockIdx.x % kSlots].v[tid], total));
   }
   if (c.patch) {
     const int x = blockIdx.x % kSlots;
@@ -505,7 +505,7 @@ __device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl
 // into the stream's white balance state; the reads clear the scratch.
 __device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s, AwbFrameLds& fl, const AwbCtx& c) {
   const int tid = threadIdx.x;
-  if (tid < 4) fl.sums[tid] = take_slots(&s->awb.sum[0][tid], 4);
+  if (tid < 4) fl.sums[tid] = take_slots(&s->awb.sum[0].v[tid], kLineSumsStride);
   if (c.patch) {
     for (int i = tid; i < 3 * kAwbBins; i += kThreads) {
       fl.hist[i] = take_slots(&s->awb.hist[0][i / kAwbBins][i % kAwbBins], 3 * kAwbBins);
@@ -1168,7 +1168,7 @@ __device__ __forceinline__ unsigned int main_flush(MainScratch* m, MainFrameLds&
   if (c.want_stats && tid < 4) {
     unsigned long long total = 0;
     for (int w = 0; w < kBlock / 64; ++w) total += fl.part[w][tid];
-    if (total != 0) token ^= fold(atomicAdd(&m->sum[blockIdx.x % kSlots][tid], total));
+    if (total != 0) token ^= fold(atomicAdd(&m->sum[blockIdx.x % kSlots].v[tid], total));
   }
   if (c.want_hist) {
     const int bins = c.bins;
@@ -1185,18 +1185,19 @@ __device__ __forceinline__ unsigned int main_flush(MainScratch* m, MainFrameLds&
 // The frame's last contributor: the histograms and the statistics vector
 // (finish_stats, isp_math.hpp:387-425); the reads clear the scratch. With a
 // device exposure state, the frame runs at the state's exposure and leaves its
-// suggestion there for the stream's next frame.
+// suggestion there for the stream's next frame. The first wave runs the serial
+// epilogue while the other waves gather the histograms.
 template <int kBlock = kThreads>
 __device__ __forceinline__ void main_finish(const FrameDesc& f, MainScratch* m, MainFrameLds& fl, const MainCtx& c) {
   const int tid = threadIdx.x;
-  if (c.want_hist) {
+  if (tid < 4) fl.sums[tid] = take_slots(&m->sum[0].v[tid], kLineSumsStride);
+  __syncthreads();
+  if (c.want_hist && tid >= 64) {
     const int bins = c.bins;
-    for (int i = tid; i < 4 * bins; i += kBlock) {
+    for (int i = tid - 64; i < 4 * bins; i += kBlock - 64) {
       f.histogram[i] = take_slots(&m->hist[0][i / bins][i % bins], 4 * kMaxHistBins);
     }
   }
-  if (tid < 4) fl.sums[tid] = take_slots(&m->sum[0][tid], 4);
-  __syncthreads();
   if (c.want_stats && tid == 0) {
     const unsigned long long sums[4] = {fl.sums[0], fl.sums[1], fl.sums[2], fl.sums[3]};
     long long exposure_ns = f.exposure_ns;
@@ -1397,10 +1398,10 @@ __global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* _
       if (cur >= 0) {
         const BatchFrame& pb = table->frames[cur];
         const unsigned int token = awb_flush(&pb.s->awb, fl, ac, acc);
-        if (complete_items(&pb.s->awb.done[0], items_done, static_cast<unsigned int>(pb.awb_items), token,
+        if (complete_items(&pb.s->awb.done[0].n, items_done, static_cast<unsigned int>(pb.awb_items), token,
                            &fl.flag)) {
           awb_finish(pb.f, pb.s, fl, ac);
-          if (threadIdx.x == 0) atomicExch(&pb.s->awb.done[0], 0U);
+          if (threadIdx.x == 0) atomicExch(&pb.s->awb.done[0].n, 0U);
         }
         __syncthreads();  // the epilogue is done with the frame LDS
 #pragma unroll
@@ -1429,9 +1430,9 @@ __global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* _
   if (cur >= 0) {
     const BatchFrame& pb = table->frames[cur];
     const unsigned int token = awb_flush(&pb.s->awb, fl, ac, acc);
-    if (complete_items(&pb.s->awb.done[0], items_done, static_cast<unsigned int>(pb.awb_items), token, &fl.flag)) {
+    if (complete_items(&pb.s->awb.done[0].n, items_done, static_cast<unsigned int>(pb.awb_items), token, &fl.flag)) {
       awb_finish(pb.f, pb.s, fl, ac);
-      if (threadIdx.x == 0) atomicExch(&pb.s->awb.done[0], 0U);
+      if (threadIdx.x == 0) atomicExch(&pb.s->awb.done[0].n, 0U);
     }
   }
   q.leave();
@@ -1468,9 +1469,9 @@ __device__ __forceinline__ void batch_main_body(const BatchTable* __restrict__ t
   auto close = [&]() {
     const BatchFrame& pb = table->frames[cur];
     const unsigned int token = main_flush(&pb.s->main, fl, mc, acc);
-    if (complete_items(&pb.s->main.done[0], items_done, static_cast<unsigned int>(pb.items), token, &fl.flag)) {
+    if (complete_items(&pb.s->main.done[0].n, items_done, static_cast<unsigned int>(pb.items), token, &fl.flag)) {
       main_finish(pb.f, &pb.s->main, fl, mc);
-      if (threadIdx.x == 0) atomicExch(&pb.s->main.done[0], 0U);
+      if (threadIdx.x == 0) atomicExch(&pb.s->main.done[0].n, 0U);
     }
   };
   for (int item = q.first(); item < q.total; item = q.next_item(item)) {

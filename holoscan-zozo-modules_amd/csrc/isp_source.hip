@@ -50,14 +50,16 @@ struct alignas(16) P4 {
 };
 
 // The scratch of isp_check_frame_device: the rebuilt histograms and the error
-// sum in kSlots copies each (block b adds into copy b % kSlots), and the
-// completion counters. Zero between checks.
+// sum in kSlots copies each (block b adds into copy b % kSlots; each error sum
+// alone on its line), and the completion counters. Zero between checks.
+struct alignas(256) LineTotal {
+  unsigned long long v;
+};
 struct CheckScratch {
-  unsigned long long err[kSlots];
+  LineTotal err[kSlots];
   unsigned int hist[kSlots][kCheckEntries];
-  unsigned int done[kSlots];
-  unsigned int top;
-  unsigned int pad[7];
+  LineCounter done[kSlots];
+  LineCounter top;
 };
 static_assert(sizeof(CheckScratch) == ISP_FRAME_CHECK_SCRATCH_BYTES, "scratch size is part of the ABI");
 
@@ -226,7 +228,7 @@ __global__ void __launch_bounds__(kSrcThreads) check_kernel(const Chart s, const
   if (tid == 0) {
     unsigned long long block = 0ULL;
     for (int w = 0; w < kSrcThreads / 64; ++w) block += lds.wave_err[w];
-    if (block != 0ULL) token ^= fold(atomicAdd(&scratch->err[slot], block));
+    if (block != 0ULL) token ^= fold(atomicAdd(&scratch->err[slot].v, block));
   }
   for (int i = tid; i < kCheckEntries; i += kSrcThreads) {
     const unsigned int c = lds.bins[i];
@@ -246,7 +248,7 @@ __global__ void __launch_bounds__(kSrcThreads) check_kernel(const Chart s, const
   if (tid == 0) {
     unsigned int total = 0U;
     for (int w = 0; w < kSrcThreads / 64; ++w) total += lds.wave_miss[w];
-    result->squared_error = take_slots(&scratch->err[0], 1);
+    result->squared_error = take_slots(&scratch->err[0].v, sizeof(LineTotal) / sizeof(unsigned long long));
     result->histogram_mismatches = total;
     result->reserved = 0U;
   }

@@ -184,7 +184,7 @@ class FrameCheckResult(C.Structure):
                 ("reserved", C.c_uint32), ("stats", C.c_float * STATS_SIZE)]
 
 
-FRAME_CHECK_SCRATCH_BYTES = 32896
+FRAME_CHECK_SCRATCH_BYTES = 37120
 
 P = C.POINTER
 _void = C.c_void_p

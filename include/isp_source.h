@@ -37,7 +37,7 @@ typedef struct isp_frame_check_result {
 
 /* Device scratch of isp_check_frame_device: zero it once when it is allocated;
  * every check leaves it zeroed again. */
-#define ISP_FRAME_CHECK_SCRATCH_BYTES 32896
+#define ISP_FRAME_CHECK_SCRATCH_BYTES 37120
 
 /* render_synthetic_raw_cuda (synthetic_raw_cuda.hpp:17-19): the frame of
  * `config` into device memory at `data`, rows `pitch` bytes apart (at least
