@@ -7,10 +7,13 @@
 //               Persistent blocks walk 64x64 tiles; each thread takes kC/2
 //               complete 2x2 quads from two coalesced row-chunk loads, skips
 //               quads with a saturated sample, and sums the linearised values
-//               (2^-20 units) per colour in registers; white patch adds an
-//               LDS-privatised 3 x 1024 histogram. The last block to finish
-//               turns the sums into gains (finish_white_balance,
-//               isp_math.hpp:357-384) in the pipeline's device state.
+//               (2^-20 units) per quad position in registers, in straight-line
+//               code for whole chunks; white patch adds an LDS-privatised
+//               3 x 1024 histogram. Each block gives the positions their
+//               colours once, when it adds its sums to the frame's scratch.
+//               The last block to finish turns the sums into gains
+//               (finish_white_balance, isp_math.hpp:357-384) in the
+//               pipeline's device state.
 //
 //  main_kernel  persistent blocks walk 64 x 32 output tiles:
 //               1. every thread first issues its coalesced row-chunk loads
@@ -336,6 +339,9 @@ __device__ float percentile_wave(const unsigned int* hist, float percentile) {
 }
 
 constexpr int kAwbTile = kBatchAwbTile;
+// A thread's white balance accumulators: the fixed-point sums of the four quad
+// positions, then the count of complete quads.
+constexpr int kAwbAcc = 5;
 
 // LDS of the white balance pass: per frame, and per tile.
 struct AwbFrameLds {
@@ -374,67 +380,156 @@ __device__ __forceinline__ void awb_begin(const FrameDesc& f, AwbFrameLds& fl, A
   c.lsc = f.lsc != nullptr;
 }
 
-// One 64 x 64 sample tile: white_balance_quad (isp_math.hpp:311-332) over its
-// complete quads, sums into local[0..2], quads into local[3].
+// A thread's task in a 64 x 64 sample tile: one row chunk of kC samples in
+// each of the two rows of a quad row (kC / 2 complete quads).
 template <int kFmt>
-__device__ __forceinline__ void awb_tile(const FrameDesc& f, const AwbCtx& c, AwbFrameLds& fl, AwbTileLds& tl,
-                                         int tile, int tiles_x, int fast_rows, unsigned long long local[4]) {
+struct AwbTask {
+  const std::uint8_t* row0;
+  int x0, y0, cx, qy;
+  bool active, whole;
+};
+
+template <int kFmt>
+__device__ __forceinline__ AwbTask<kFmt> awb_task(const FrameDesc& f, int tile, int tiles_x, int fast_rows) {
   constexpr int kC = Chunk<kFmt>::kSamples;
   constexpr int kTasksX = kAwbTile / kC;
   constexpr int kTasks = kTasksX * (kAwbTile / 2);
   static_assert(kTasks <= kThreads, "one task per thread");
   const int tid = threadIdx.x;
-  const int W = f.width;
-  const int H = f.height;
-  const int qw = W / 2;
-  const int qh = H / 2;
+  const int qw = f.width / 2;
+  const int qh = f.height / 2;
+  AwbTask<kFmt> t;
   const int tyi = tile / tiles_x;
-  const int x0 = (tile - tyi * tiles_x) * kAwbTile;
-  const int y0 = tyi * kAwbTile;
+  t.x0 = (tile - tyi * tiles_x) * kAwbTile;
+  t.y0 = tyi * kAwbTile;
   const int qr = tid / kTasksX;
-  const int cx = x0 + (tid - qr * kTasksX) * kC;
-  const int qy = (y0 >> 1) + qr;
-  const bool active = tid < kTasks && qy < qh && cx < 2 * qw;
-  const bool whole = active && fast_rows != 0 && cx + kC <= W;
-  const std::uint8_t* row0 = f.raw + static_cast<long long>(2 * qy) * f.raw_pitch;
-  const std::uint8_t* row1 = row0 + f.raw_pitch;
-  Packed<kFmt> pk[2];
-  if (whole) {
-    load_packed<kFmt>(row0, cx, pk[0]);
-    load_packed<kFmt>(row1, cx, pk[1]);
+  t.cx = t.x0 + (tid - qr * kTasksX) * kC;
+  t.qy = (t.y0 >> 1) + qr;
+  t.active = tid < kTasks && t.qy < qh && t.cx < 2 * qw;
+  t.whole = t.active && fast_rows != 0 && t.cx + kC <= f.width;
+  t.row0 = f.raw + static_cast<long long>(2 * t.qy) * f.raw_pitch;
+  return t;
+}
+
+// A tile's coalesced chunk loads (whole chunks only; the edges read byte by
+// byte in awb_tile).
+template <int kFmt>
+__device__ __forceinline__ void awb_load(const FrameDesc& f, const AwbTask<kFmt>& t, Packed<kFmt> (&pk)[2]) {
+  if (t.whole) {
+    load_packed<kFmt>(t.row0, t.cx, pk[0]);
+    load_packed<kFmt>(t.row0 + f.raw_pitch, t.cx, pk[1]);
   }
+}
+
+// The common case of a tile's task: a whole chunk, gray-world sums, lens
+// shading from the tile's table (or none). Straight-line code: quad position p =
+// 2 r + (column & 1) is known per sample (chunks start at even columns), the
+// chunk's table entries come in 128-bit LDS reads, and a whole chunk's quads
+// all lie inside the frame. Same arithmetic as awb_tile's general path.
+template <int kFmt, bool kLsc>
+__device__ __forceinline__ void awb_chunk_fast(const FrameDesc& f, const AwbCtx& c, AwbTileLds& tl,
+                                               const AwbTask<kFmt>& t, const Packed<kFmt> (&pk)[2], int lo,
+                                               unsigned long long local[kAwbAcc]) {
+  constexpr int kC = Chunk<kFmt>::kSamples;
+  int raw[2][kC];
+  unpack<kFmt>(pk[0], raw[0]);
+  unpack<kFmt>(pk[1], raw[1]);
+  float gain[2][kC];
+  if constexpr (kLsc) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      float ty;
+      const int seg = lsc_row(f, 2 * t.qy + r, &ty) - lo;
+      const F4* e = reinterpret_cast<const F4*>(&tl.lsc.e[seg][r][t.cx - t.x0]);
+#pragma unroll
+      for (int h = 0; h < kC / 2; ++h) {
+        const F4 v = e[h];
+        gain[r][2 * h] = v.v[0] + v.v[1] * ty;
+        gain[r][2 * h + 1] = v.v[2] + v.v[3] * ty;
+      }
+    }
+  }
+  unsigned int pos[4] = {0U, 0U, 0U, 0U};
+  unsigned int quads = 0U;
+#pragma unroll
+  for (int q = 0; q < kC / 2; ++q) {
+    const bool ok = raw[0][2 * q] < f.white && raw[0][2 * q + 1] < f.white && raw[1][2 * q] < f.white &&
+                    raw[1][2 * q + 1] < f.white;
+    unsigned int fx[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int r = k >> 1;
+      const int col = 2 * q + (k & 1);
+      float v = (static_cast<float>(raw[r][col]) - c.qc.black[k]) * c.qc.inv[k];
+      v = v > 0.0F ? v : 0.0F;
+      if constexpr (kLsc) v = v * gain[r][col];
+      v = v < 4.0F ? v : 4.0F;
+      fx[k] = static_cast<unsigned int>(v * 1048576.0F + 0.5F);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pos[k] += ok ? fx[k] : 0U;
+    quads += ok ? 1U : 0U;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) local[k] += pos[k];
+  local[4] += quads;
+}
+
+// One 64 x 64 sample tile, its chunks loaded by awb_load: white_balance_quad
+// (isp_math.hpp:311-332) over its complete quads: the fixed-point values add up
+// per quad position into local[0..3] (exact integer sums; awb_flush gives them
+// their colours), complete quads into local[4].
+template <int kFmt>
+__device__ __forceinline__ void awb_tile(const FrameDesc& f, const AwbCtx& c, AwbFrameLds& fl, AwbTileLds& tl,
+                                         const AwbTask<kFmt>& t, const Packed<kFmt> (&pk)[2],
+                                         unsigned long long local[kAwbAcc]) {
+  constexpr int kC = Chunk<kFmt>::kSamples;
+  const int H = f.height;
+  const int qw = f.width / 2;
   int lo = 0;
   bool table = false;
   __syncthreads();  // the previous tile's table readers are done
   if (c.lsc) {
-    lo = lsc_row(f, y0);
-    table = lsc_row(f, min(y0 + kAwbTile - 1, H - 1)) - lo <= 1;
-    if (table) build_lsc_table<kAwbTile>(f, c.grid, x0, lo, tl.lsc);
+    lo = lsc_row(f, t.y0);
+    table = lsc_row(f, min(t.y0 + kAwbTile - 1, H - 1)) - lo <= 1;
+    if (table) build_lsc_table<kAwbTile>(f, c.grid, t.x0, lo, tl.lsc);
   }
   __syncthreads();
-  if (!active) return;
+  if (!t.active) return;
+  // Every tile has the table where the frame's grid rows allow it; the test is
+  // made uniform so that the choice below is a scalar branch.
+  table = __builtin_amdgcn_readfirstlane(static_cast<int>(table)) != 0;
+  if (t.whole && (table || !c.lsc) && !c.patch) {
+    if (c.lsc) {
+      awb_chunk_fast<kFmt, true>(f, c, tl, t, pk, lo, local);
+    } else {
+      awb_chunk_fast<kFmt, false>(f, c, tl, t, pk, lo, local);
+    }
+    return;
+  }
+  const std::uint8_t* row1 = t.row0 + f.raw_pitch;
   int raw[2][kC];
-  if (whole) {
+  if (t.whole) {
     unpack<kFmt>(pk[0], raw[0]);
     unpack<kFmt>(pk[1], raw[1]);
   } else {
 #pragma unroll
     for (int k = 0; k < kC; ++k) {
-      raw[0][k] = cx + k < 2 * qw ? load_raw(row0, kFmt, cx + k) : 0;
-      raw[1][k] = cx + k < 2 * qw ? load_raw(row1, kFmt, cx + k) : 0;
+      raw[0][k] = t.cx + k < 2 * qw ? load_raw(t.row0, kFmt, t.cx + k) : 0;
+      raw[1][k] = t.cx + k < 2 * qw ? load_raw(row1, kFmt, t.cx + k) : 0;
     }
   }
   float tyr[2] = {0.0F, 0.0F};
   int seg[2] = {0, 0};
   if (table) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) seg[r] = lsc_row(f, 2 * qy + r, &tyr[r]) - lo;
+    for (int r = 0; r < 2; ++r) seg[r] = lsc_row(f, 2 * t.qy + r, &tyr[r]) - lo;
   }
-  unsigned int sum[3] = {0U, 0U, 0U};
+  unsigned int pos[4] = {0U, 0U, 0U, 0U};
   unsigned int quads = 0U;
 #pragma unroll
   for (int q = 0; q < kC / 2; ++q) {
-    if ((cx >> 1) + q >= qw) continue;
+    if ((t.cx >> 1) + q >= qw) continue;
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < 4; ++k) ok = ok && raw[k >> 1][2 * q + (k & 1)] < f.white;
@@ -450,39 +545,57 @@ __device__ __forceinline__ void awb_tile(const FrameDesc& f, const AwbCtx& c, Aw
       if (c.lsc) {
         float gain;
         if (table) {
-          const F2 e = tl.lsc.e[seg[r]][r][cx - x0 + col];
+          const F2 e = tl.lsc.e[seg[r]][r][t.cx - t.x0 + col];
           gain = e.v[0] + e.v[1] * tyr[r];
         } else {
-          gain = lsc_gain(f, cx + col, 2 * qy + r, cfa_channel(f.cfa, k));
+          gain = lsc_gain(f, t.cx + col, 2 * t.qy + r, cfa_channel(f.cfa, k));
         }
         v = v * gain;
       }
       v = v < 4.0F ? v : 4.0F;
-      const int colour = c.colour[k];
-      const unsigned int fixed = static_cast<unsigned int>(v * 1048576.0F + 0.5F);
-      sum[0] += colour == 0 ? fixed : 0U;
-      sum[1] += colour == 1 ? fixed : 0U;
-      sum[2] += colour == 2 ? fixed : 0U;
+      pos[k] += static_cast<unsigned int>(v * 1048576.0F + 0.5F);
       if (c.patch) {
         const int b = static_cast<int>(v * static_cast<float>(kAwbBins));
-        atomicAdd(&fl.hist[colour * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
+        atomicAdd(&fl.hist[c.colour[k] * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
       }
     }
   }
 #pragma unroll
-  for (int k = 0; k < 3; ++k) local[k] += sum[k];
-  local[3] += quads;
+  for (int k = 0; k < 4; ++k) local[k] += pos[k];
+  local[4] += quads;
+}
+
+// The tiles tile, tile + stride, ... below `end` of one frame. (Loading the
+// next tile's chunks while one is summed measured no faster.)
+template <int kFmt>
+__device__ __forceinline__ void awb_tiles(const FrameDesc& f, const AwbCtx& c, AwbFrameLds& fl, AwbTileLds& tl,
+                                          int tile, int end, int stride, int tiles_x, int fast_rows,
+                                          unsigned long long local[kAwbAcc]) {
+  for (; tile < end; tile += stride) {
+    const AwbTask<kFmt> t = awb_task<kFmt>(f, tile, tiles_x, fast_rows);
+    Packed<kFmt> pk[2];
+    awb_load<kFmt>(f, t, pk);
+    awb_tile<kFmt>(f, c, fl, tl, t, pk, local);
+  }
 }
 
 // Adds the block's partial sums and histograms to the frame's scratch; returns
 // the token of the atomics (see complete_grid()).
 __device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl, const AwbCtx& c,
-                                                  const unsigned long long local[4]) {
+                                                  const unsigned long long local[kAwbAcc]) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
+  // The quad positions' sums by colour (R, G, B), and the quad count.
+  unsigned long long sums[4] = {0, 0, 0, local[4]};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const unsigned long long v = wave_sum(local[k]);
+    sums[0] += c.colour[k] == 0 ? local[k] : 0ULL;
+    sums[1] += c.colour[k] == 1 ? local[k] : 0ULL;
+    sums[2] += c.colour[k] == 2 ? local[k] : 0ULL;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const unsigned long long v = wave_sum(sums[k]);
     if ((tid & 63) == 0) fl.part[wave][k] = v;
   }
   __syncthreads();
@@ -559,8 +672,8 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
   __shared__ __attribute__((aligned(16))) AwbTileLds tl;
   AwbCtx c;
   awb_begin(f, fl, c);
-  unsigned long long local[4] = {0, 0, 0, 0};
-  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) awb_tile<kFmt>(f, c, fl, tl, tile, tiles_x, fast_rows, local);
+  unsigned long long local[kAwbAcc] = {0, 0, 0, 0, 0};
+  awb_tiles<kFmt>(f, c, fl, tl, blockIdx.x, tiles, gridDim.x, tiles_x, fast_rows, local);
   const unsigned int token = awb_flush(&s->awb, fl, c, local);
   if (!complete_grid(s->awb.done, &s->awb.top, token, &fl.flag)) return;
   awb_finish(f, s, fl, c);
@@ -1293,10 +1406,10 @@ __device__ __forceinline__ void batch_main_item(const BatchFrame& b, const MainC
 
 template <int kFmt>
 __device__ __forceinline__ void batch_awb_item(const BatchFrame& b, const AwbCtx& c, AwbFrameLds& fl,
-                                               AwbTileLds& tl, int item, unsigned long long acc[4]) {
+                                               AwbTileLds& tl, int item, unsigned long long acc[kAwbAcc]) {
   const int first = item * b.awb_tiles_per_item;
   const int end = min(first + b.awb_tiles_per_item, b.awb_tiles);
-  for (int t = first; t < end; ++t) awb_tile<kFmt>(b.f, c, fl, tl, t, b.awb_tiles_x, b.fast_rows, acc);
+  awb_tiles<kFmt>(b.f, c, fl, tl, first, end, 1, b.awb_tiles_x, b.fast_rows, acc);
 }
 
 // The frame (in the pass's order) that holds item `item` of a pass, by one
@@ -1387,7 +1500,7 @@ __global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* _
   const int count = table->count;
   int cur = -1;
   unsigned int items_done = 0;
-  unsigned long long acc[4] = {0, 0, 0, 0};
+  unsigned long long acc[kAwbAcc] = {0, 0, 0, 0, 0};
   AwbCtx ac;
   for (int item = q.first(); item < q.total; item = q.next_item(item)) {
     q.prefetch();
@@ -1405,7 +1518,7 @@ __global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* _
         }
         __syncthreads();  // the epilogue is done with the frame LDS
 #pragma unroll
-        for (int k = 0; k < 4; ++k) acc[k] = 0;
+        for (int k = 0; k < kAwbAcc; ++k) acc[k] = 0;
         items_done = 0;
       }
       cur = f;
