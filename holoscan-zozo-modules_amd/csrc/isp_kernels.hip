@@ -853,6 +853,66 @@ __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains
   c.grid = stage_grid(f, fl.grid);
 }
 
+// Lens shading of a tile's samples: none, from the tile's table, or per sample
+// (tiles whose rows span more than two grid rows).
+constexpr int kLscNone = 0;
+constexpr int kLscTable = 1;
+constexpr int kLscSlow = 2;
+
+// One row chunk's preprocessing constants: black level, 1 / range and colour
+// gain of its even and odd columns, and where it lies.
+struct PreRow {
+  float b0, b1, i0, i1, g0, g1;
+  int cx, y, par, ci;
+  bool count, interior_x;
+};
+
+// preprocess (isp_math.hpp:151-155) of one row chunk into sample-tile row
+// `out`: (raw - black) / range, cut at 0, lens shading, white balance, cut at 1;
+// saturated samples of the tile's own rows and columns counted. With the table,
+// the chunk's kC entries come first in 128-bit LDS reads, so that no sample
+// waits on its own read.
+template <int kFmt, int kLsc, int kPad, int kSW>
+__device__ __forceinline__ void pre_chunk(const FrameDesc& f, const PreRow& r, const int (&raw)[Chunk<kFmt>::kSamples],
+                                          const F2* table, float tyr, float* out, unsigned int& saturated) {
+  constexpr int kC = Chunk<kFmt>::kSamples;
+  float gain[kC];
+  if constexpr (kLsc == kLscTable) {
+    const F4* e = reinterpret_cast<const F4*>(table);
+#pragma unroll
+    for (int h = 0; h < kC / 2; ++h) {
+      const F4 v = e[h];
+      gain[2 * h] = v.v[0] + v.v[1] * tyr;
+      gain[2 * h + 1] = v.v[2] + v.v[3] * tyr;
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < kC / 4; ++h) {
+    const int col0 = kPad - kC + r.ci * kC + 4 * h;
+    if (col0 < 0 || col0 + 4 > kSW) continue;
+    F4 o;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int k = 4 * h + m;
+      const bool odd = (m & 1) != 0;  // cx and col0 are even
+      float v = (static_cast<float>(raw[k]) - (odd ? r.b1 : r.b0)) * (odd ? r.i1 : r.i0);
+      v = v > 0.0F ? v : 0.0F;
+      if constexpr (kLsc == kLscTable) {
+        v = v * gain[k];
+      } else if constexpr (kLsc == kLscSlow) {
+        const int x = r.cx + k;
+        const float g = x >= 0 && x < f.width ? lsc_gain(f, x, r.y, cfa_channel(f.cfa, (r.par << 1) | (x & 1))) : 1.0F;
+        v = v * g;
+      }
+      v = v * (odd ? r.g1 : r.g0);
+      v = v < 1.0F ? v : 1.0F;
+      o.v[m] = v;
+      if (r.count && (r.interior_x || r.cx + k < f.width)) saturated += raw[k] >= f.white ? 1U : 0U;
+    }
+    *reinterpret_cast<F4*>(&out[col0]) = o;
+  }
+}
+
 // One kTileH x 64 output tile; statistics sums into acc[0..2], saturated
 // samples into acc[3].
 template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
@@ -907,6 +967,9 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
   if (c.lsc) {
     lo = lsc_row(f, sy0 > 0 ? sy0 : 0);
     table = lsc_row(f, sy0 + kSH - 1 < H - 1 ? sy0 + kSH - 1 : H - 1) - lo <= 1;
+    // The same for every thread: made uniform so that the per-sample choice
+    // below is a scalar branch.
+    table = __builtin_amdgcn_readfirstlane(static_cast<int>(table)) != 0;
     if (table) build_lsc_table<kLscCols>(f, c.grid, cx0, lo, tl.lsc);
     __syncthreads();
   }
@@ -937,38 +1000,17 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
     const float i1 = par ? c.qc.inv[3] : c.qc.inv[1];
     const float g0 = par ? c.qc.gain[2] : c.qc.gain[0];
     const float g1 = par ? c.qc.gain[3] : c.qc.gain[1];
-    float tyr = 0.0F;
-    int seg = 0;
-    if (table) seg = lsc_row(f, y, &tyr) - lo;
     const bool count = c.want_stats && yv >= y0 && yv < y0 + kTileH && yv < H && ci >= 1 && ci <= 64 / kC;
-#pragma unroll
-    for (int h = 0; h < kC / 4; ++h) {
-      const int col0 = kPad - kC + ci * kC + 4 * h;
-      if (col0 < 0 || col0 + 4 > kSW) continue;
-      F4 o;
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int k = 4 * h + m;
-        const bool odd = (m & 1) != 0;  // cx and col0 are even
-        float v = (static_cast<float>(raw[k]) - (odd ? b1 : b0)) * (odd ? i1 : i0);
-        v = v > 0.0F ? v : 0.0F;
-        if (c.lsc) {
-          float gain;
-          if (table) {
-            const F2 e = tl.lsc.e[seg][par][ci * kC + k];
-            gain = e.v[0] + e.v[1] * tyr;
-          } else {
-            const int x = cx + k;
-            gain = x >= 0 && x < W ? lsc_gain(f, x, y, cfa_channel(f.cfa, (par << 1) | (x & 1))) : 1.0F;
-          }
-          v = v * gain;
-        }
-        v = v * (odd ? g1 : g0);
-        v = v < 1.0F ? v : 1.0F;
-        o.v[m] = v;
-        if (count && (interior_x || cx + k < W)) saturated += raw[k] >= f.white ? 1U : 0U;
-      }
-      *reinterpret_cast<F4*>(&samp[sr * kSW + col0]) = o;
+    const PreRow row{b0, b1, i0, i1, g0, g1, cx, y, par, ci, count, interior_x};
+    float* out = &samp[sr * kSW];
+    if (!c.lsc) {
+      pre_chunk<kFmt, kLscNone, kPad, kSW>(f, row, raw, nullptr, 0.0F, out, saturated);
+    } else if (table) {
+      float tyr;
+      const int seg = lsc_row(f, y, &tyr) - lo;
+      pre_chunk<kFmt, kLscTable, kPad, kSW>(f, row, raw, &tl.lsc.e[seg][par][ci * kC], tyr, out, saturated);
+    } else {
+      pre_chunk<kFmt, kLscSlow, kPad, kSW>(f, row, raw, nullptr, 0.0F, out, saturated);
     }
   }
   acc[3] += saturated;
