@@ -39,6 +39,8 @@ def parse():
                     help="real-time length of the 32-camera serving trace (0: skip)")
     ap.add_argument("--source-launches", type=int, default=50,
                     help="timed launches of the device raw source and the frame checker (0: skip)")
+    ap.add_argument("--color-launches", type=int, default=100,
+                    help="timed launches of each 4K colour conversion after the ISP (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times): the reference "
                          "point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -384,6 +386,43 @@ def source_and_check(launches):
     return out
 
 
+def color_conversion(launches):
+    """The conversion after the ISP (isp_color_convert_device) on a 4K frame: each kernel's
+    average launch time from HIP events over `launches` launches, and its rate against the
+    bytes it must move (read the RGB frame once, write the YCbCr planes once)."""
+    import torch
+    import numpy as np
+    from holoscan_isp import _abi, color
+    stream = torch.cuda.current_stream()
+    out = {"workload": "3840x2160, BT.709 limited range, seeded RGB, device buffers", "launches": launches}
+    w, h = 3840, 2160
+    for name, src_fmt, dst_fmt in (("rgb8_to_nv12", _abi.PIX_RGB8, _abi.PIX_NV12),
+                                   ("rgba8_to_nv12", _abi.PIX_RGBA8, _abi.PIX_NV12),
+                                   ("rgba8_to_i420", _abi.PIX_RGBA8, _abi.PIX_I420),
+                                   ("rgb8_to_yuyv", _abi.PIX_RGB8, _abi.PIX_YUYV)):
+        src = color.Picture(src_fmt, w, h, device=True)
+        src.buf.copy_(torch.from_numpy(np.random.default_rng(1).integers(0, 256, src.nbytes, dtype=np.uint8)))
+        dst = color.Picture(dst_fmt, w, h, device=True)
+        sv, dv = src.view(), dst.view()
+
+        def run():
+            color.convert_device(sv, dv, _abi.MATRIX_BT709, _abi.RANGE_LIMITED, stream.cuda_stream)
+
+        for _ in range(3):
+            run()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(launches):
+            run()
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) / launches * 1e3
+        moved = src.nbytes + dst.nbytes
+        out[name] = {"us": round(us, 2), "bytes": moved, "gb_s": round(moved / us / 1e3, 1),
+                     "hbm_frac": round(moved / us / 1e3 / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -499,6 +538,8 @@ def main():
         result["served_32_cameras"] = served_report(args.served_seconds)
     if args.source_launches > 0:
         result["device_source_and_check"] = source_and_check(args.source_launches)
+    if args.color_launches > 0:
+        result["color_conversion_4k"] = color_conversion(args.color_launches)
     if not args.no_cpu:
         base = cpu_baseline(frame, params, out_format, args.cpu_frames)
         result["cpu_baseline"] = base["single"]

@@ -1,4 +1,5 @@
-"""ctypes mirror of include/isp_c.h, include/isp_synthetic.h, include/isp_source.h and include/isp_op.h.
+"""ctypes mirror of include/isp_c.h, include/isp_synthetic.h, include/isp_source.h, include/isp_color.h
+and include/isp_op.h.
 
 The field order and types follow the C headers exactly; the names follow the
 reference's holoscan::isp types (modules/isp/include/holoscan/isp/isp_params.hpp).
@@ -186,6 +187,24 @@ class FrameCheckResult(C.Structure):
 
 FRAME_CHECK_SCRATCH_BYTES = 37120
 
+# include/isp_color.h
+(PIX_NV12, PIX_NV16, PIX_I420, PIX_YUYV, PIX_UYVY, PIX_RGB8, PIX_BGR8, PIX_RGBA8, PIX_BGRA8, PIX_MONO8,
+ PIX_MONO16, PIX_RGB16) = range(12)
+MATRIX_BT601, MATRIX_BT709, MATRIX_BT2020 = range(3)
+RANGE_LIMITED, RANGE_FULL = range(2)
+
+
+class PlaneView(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("pitch", C.c_size_t)]
+
+
+class ImageView(C.Structure):
+    _fields_ = [("format", C.c_uint8), ("width", C.c_int32), ("height", C.c_int32), ("planes", PlaneView * 3)]
+
+
+class ColorParams(C.Structure):
+    _fields_ = [("matrix", C.c_uint8), ("range", C.c_uint8)]
+
 P = C.POINTER
 _void = C.c_void_p
 
@@ -264,6 +283,9 @@ PROTOTYPES = [
     ("isp_check_frame_device", C.c_int, [P(SyntheticConfig), _void, C.c_int32, _void, C.c_size_t, _void,
                                          _void, _void, _void, _void]),
     ("isp_frame_check_psnr", C.c_double, [P(FrameCheckResult), C.c_int32, C.c_int32]),
+    ("isp_color_params_init", None, [P(ColorParams)]),
+    ("isp_color_plane_size", C.c_int, [C.c_uint8, C.c_int32, C.c_int32, C.c_int32, P(C.c_size_t), P(C.c_int32)]),
+    ("isp_color_convert_device", C.c_int, [P(ImageView), P(ImageView), P(ColorParams), _void]),
 ]
 
 

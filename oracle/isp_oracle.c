@@ -24,6 +24,9 @@
  *    gain 1.5), histogram rebuild from the image, black frame.
  *  - the 1024-bin histogram extension (no reference counterpart) is pinned by
  *    rebuilding it from the written RGB16 / RGB32F image only.
+ *  - the colour conversion (oracle_color_convert) against the published colour
+ *    bars (ITU-R BT.801, SMPTE RP 219, JFIF) and the module's own CPU converter
+ *    compiled from its sources (oracle/_ref), byte for byte.
  *  - the frame checker (oracle_frame_check) by the reference demo's published
  *    run (applications/isp/README.md:89: mean and worst PSNR, mean luma and AE
  *    error of 30 checked 640x480 frames) and by closed forms: a perfect image
@@ -765,4 +768,117 @@ void oracle_frame_check(const float* truth, int width, int height, const float* 
 double oracle_frame_psnr(const isp_frame_check_result* result, int width, int height) {
   const double squared = (double)result->squared_error;
   return squared == 0.0 ? 99.0 : 10.0 * log10(255.0 * 255.0 * 3.0 * (double)width * (double)height / squared);
+}
+
+/* ---- colour conversion (modules/color-convert) --------------------------- */
+
+/* from_rgb (conversion.cpp:176-198) as floats, as make_plan rounds its map
+ * (conversion.cpp:428-436): c_i = m[3i] R + m[3i+1] G + m[3i+2] B + o[i]. */
+static void color_affine(int gray, int matrix, int range, float m[9], float o[3]) {
+  double kr = 0.299, kb = 0.114;
+  if (matrix == ISP_MATRIX_BT709) {
+    kr = 0.2126;
+    kb = 0.0722;
+  } else if (matrix == ISP_MATRIX_BT2020) {
+    kr = 0.2627;
+    kb = 0.0593;
+  }
+  const double kg = 1.0 - kr - kb;
+  double a[3][4];
+  memset(a, 0, sizeof(a));
+  if (gray) {
+    a[0][0] = kr;
+    a[0][1] = kg;
+    a[0][2] = kb;
+  } else {
+    const int limited = range == ISP_RANGE_LIMITED;
+    const double ys = limited ? 219.0 : 255.0, yo = limited ? 16.0 : 0.0, cs = limited ? 224.0 : 255.0;
+    const double cb_den = 2.0 * (1.0 - kb), cr_den = 2.0 * (1.0 - kr);
+    const double s = ys / 255.0, c = cs / 255.0;
+    a[0][0] = kr * s;
+    a[0][1] = kg * s;
+    a[0][2] = kb * s;
+    a[0][3] = yo;
+    a[1][0] = -kr / cb_den * c;
+    a[1][1] = -kg / cb_den * c;
+    a[1][2] = (1.0 - kb) / cb_den * c;
+    a[1][3] = 128.0;
+    a[2][0] = (1.0 - kr) / cr_den * c;
+    a[2][1] = -kg / cr_den * c;
+    a[2][2] = -kb / cr_den * c;
+    a[2][3] = 128.0;
+  }
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) m[3 * i + j] = (float)a[i][j];
+    o[i] = (float)a[i][3];
+  }
+}
+
+/* to_u8 (kernel_math.hpp:80-83). */
+static uint8_t color_code(float v) {
+  const float r = floorf(v + 0.5F);
+  return (uint8_t)(r <= 0.0F ? 0.0F : (r >= 255.0F ? 255.0F : r));
+}
+
+int oracle_color_convert(const isp_image_view* src, const isp_image_view* dst, int matrix, int range) {
+  const int s = src->format, t = dst->format;
+  const int bpp = s == ISP_PIX_RGB8 || s == ISP_PIX_BGR8 ? 3 : s == ISP_PIX_RGBA8 || s == ISP_PIX_BGRA8 ? 4 : 0;
+  const int bgr = s == ISP_PIX_BGR8 || s == ISP_PIX_BGRA8;
+  if (bpp == 0 || (t > ISP_PIX_UYVY && t != ISP_PIX_MONO8)) return -1;
+  if (src->width != dst->width || src->height != dst->height || src->width <= 0 || src->height <= 0) return -1;
+  const int W = src->width, H = src->height;
+  float m[9], o[3];
+  color_affine(t == ISP_PIX_MONO8, matrix, range, m, o);
+  /* Chroma block of the destination (convert_block, kernel_math.hpp:335-361). */
+  const int sx = t == ISP_PIX_MONO8 ? 0 : 1;
+  const int sy = t == ISP_PIX_NV12 || t == ISP_PIX_I420 ? 1 : 0;
+  const int bw = (W + (1 << sx) - 1) >> sx, bh = (H + (1 << sy) - 1) >> sy;
+  for (int by = 0; by < bh; ++by) {
+    for (int bx = 0; bx < bw; ++bx) {
+      const int x0 = bx << sx, y0 = by << sy;
+      const int x1 = x0 + (1 << sx) < W ? x0 + (1 << sx) : W;
+      const int y1 = y0 + (1 << sy) < H ? y0 + (1 << sy) : H;
+      float cb = 0.0F, cr = 0.0F;
+      int n = 0;
+      for (int y = y0; y < y1; ++y) {
+        const uint8_t* row = src->planes[0].data + (size_t)y * src->planes[0].pitch;
+        uint8_t* out = dst->planes[0].data + (size_t)y * dst->planes[0].pitch;
+        for (int x = x0; x < x1; ++x) {
+          const uint8_t* p = row + (size_t)bpp * x;
+          const float c0 = (float)p[bgr ? 2 : 0], c1 = (float)p[1], c2 = (float)p[bgr ? 0 : 2];
+          /* apply (kernel_math.hpp:234-242) */
+          const float y_ = m[0] * c0 + m[1] * c1 + m[2] * c2 + o[0];
+          const float u_ = m[3] * c0 + m[4] * c1 + m[5] * c2 + o[1];
+          const float v_ = m[6] * c0 + m[7] * c1 + m[8] * c2 + o[2];
+          /* store (kernel_math.hpp:253-300): the luma */
+          if (t == ISP_PIX_YUYV)
+            out[4 * (x >> 1) + 2 * (x & 1)] = color_code(y_);
+          else if (t == ISP_PIX_UYVY)
+            out[4 * (x >> 1) + 1 + 2 * (x & 1)] = color_code(y_);
+          else
+            out[x] = color_code(y_);
+          cb += u_;
+          cr += v_;
+          ++n;
+        }
+      }
+      if (t == ISP_PIX_MONO8) continue;
+      const float count = (float)n;
+      const uint8_t qb = color_code(cb / count), qr = color_code(cr / count);
+      /* store_chroma (kernel_math.hpp:303-331) */
+      if (t == ISP_PIX_NV12 || t == ISP_PIX_NV16) {
+        uint8_t* q = dst->planes[1].data + (size_t)by * dst->planes[1].pitch + 2 * (size_t)bx;
+        q[0] = qb;
+        q[1] = qr;
+      } else if (t == ISP_PIX_I420) {
+        dst->planes[1].data[(size_t)by * dst->planes[1].pitch + bx] = qb;
+        dst->planes[2].data[(size_t)by * dst->planes[2].pitch + bx] = qr;
+      } else {
+        uint8_t* q = dst->planes[0].data + (size_t)by * dst->planes[0].pitch + 4 * (size_t)bx;
+        q[t == ISP_PIX_YUYV ? 1 : 0] = qb;
+        q[t == ISP_PIX_YUYV ? 3 : 2] = qr;
+      }
+    }
+  }
+  return 0;
 }

@@ -18,6 +18,7 @@
 
 #include "../include/isp_c.h"
 #include "../include/isp_source.h"
+#include "../include/isp_color.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -58,6 +59,14 @@ void oracle_frame_check(const float* truth, int width, int height, const float* 
                         const float* stats, isp_frame_check_result* result);
 /* The PSNR of a checked frame (check_op.cpp:226-228). */
 double oracle_frame_psnr(const isp_frame_check_result* result, int width, int height);
+
+/* The colour conversion module's CPU converter (reference_convert,
+ * modules/color-convert/src/reference.cpp:14-27) at one size, from an RGB
+ * layout (RGB8, BGR8, RGBA8, BGRA8) into a YCbCr one (NV12, NV16, I420, YUYV,
+ * UYVY) or MONO8, views in host memory. 0, or -1 for a pair or size it does not
+ * convert. */
+int oracle_color_convert(const isp_image_view* source, const isp_image_view* destination, int matrix,
+                         int range);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,7 @@ from holoscan_isp.params import output_channel_bytes, output_channels
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "libisp_oracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libisp_ref.so")
+CC_REF_SO = os.path.join(ROOT, "oracle", "_ref", "libcc_ref.so")
 
 P = C.POINTER
 _v = C.c_void_p
@@ -32,6 +33,12 @@ ORACLE_PROTOTYPES = [
     ("oracle_frame_check", None, [P(C.c_float), C.c_int, C.c_int, P(C.c_float), C.c_int, _v, C.c_size_t,
                                   P(C.c_uint32), P(C.c_float), P(_abi.FrameCheckResult)]),
     ("oracle_frame_psnr", C.c_double, [P(_abi.FrameCheckResult), C.c_int, C.c_int]),
+    ("oracle_color_convert", C.c_int, [P(_abi.ImageView), P(_abi.ImageView), C.c_int, C.c_int]),
+]
+
+CC_REF_PROTOTYPES = [
+    ("ref_cc_last_error", C.c_char_p, []),
+    ("ref_cc_convert", C.c_int, [P(_abi.ImageView), P(_abi.ImageView), C.c_int, C.c_int, C.c_int]),
 ]
 
 REF_PROTOTYPES = [
@@ -54,6 +61,7 @@ REF_PROTOTYPES = [
 
 _oracle = None
 _ref = None
+_cc_ref = None
 
 
 def oracle():
@@ -185,3 +193,28 @@ def frame_check(truth, lut, rgb, histogram, stats=None):
 
 def frame_psnr(result, width, height):
     return oracle().oracle_frame_psnr(C.byref(result), width, height)
+
+
+def cc_ref_available():
+    return os.path.exists(CC_REF_SO)
+
+
+def cc_ref():
+    """The colour conversion module's own CPU converter, compiled from its sources."""
+    global _cc_ref
+    if _cc_ref is None:
+        _cc_ref = _abi.bind(C.CDLL(CC_REF_SO), CC_REF_PROTOTYPES)
+    return _cc_ref
+
+
+def color_convert(source, destination, matrix, range_):
+    """The oracle's conversion of host Pictures (holoscan_isp.color.Picture)."""
+    rc = oracle().oracle_color_convert(C.byref(source.view()), C.byref(destination.view()), matrix, range_)
+    if rc != 0:
+        raise ValueError("oracle_color_convert: pair or size not converted")
+
+
+def ref_color_convert(source, destination, matrix, range_, mode=1):
+    rc = cc_ref().ref_cc_convert(C.byref(source.view()), C.byref(destination.view()), matrix, range_, mode)
+    if rc != 0:
+        raise ValueError(cc_ref().ref_cc_last_error().decode())

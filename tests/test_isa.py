@@ -9,7 +9,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "holoscan-zozo-modules_amd")
 
 _BUILT = {}
-FILES = ("isp_kernels.s", "isp_source.s")
+FILES = ("isp_kernels.s", "isp_source.s", "isp_color.s")
 
 
 def asm_text(name="isp_kernels.s"):
