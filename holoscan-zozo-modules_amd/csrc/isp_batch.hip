@@ -79,6 +79,7 @@ struct isp_batch {
   std::size_t table_bytes = 0;
   bool dirty = false;    // the host table is newer than the device's
   bool sharpen = false;  // a frame sharpens: the batch kernel with the larger LDS
+  bool wide = true;      // every frame runs 32-row tiles: the 512-thread kernels
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -117,7 +118,7 @@ struct isp_batch {
   }
 
   int resident_per_cu() const {
-    const int fit = batch_main_blocks_per_cu(sharpen);
+    const int fit = batch_main_blocks_per_cu(sharpen, wide);
     return schedule.blocks_per_cu > 0 ? std::min(schedule.blocks_per_cu, fit) : fit;
   }
 
@@ -133,6 +134,7 @@ struct isp_batch {
     t.queue = schedule.queue;
     sharpen = false;
     for (const isp_batch_frame& fr : frames) sharpen = sharpen || fr.pipeline->params.c.sharpen > 0.0F;
+    wide = true;
     std::vector<double> cost(n);
     for (int i = 0; i < n; ++i) {
       const isp_batch_frame& fr = frames[i];
@@ -144,6 +146,7 @@ struct isp_batch {
       const bool malvar = c.demosaic == ISP_DEMOSAIC_MALVAR;
       const bool sharpen = c.sharpen > 0.0F;
       const int tile_h = schedule.tile_height[fr.raw.format][malvar ? 1 : 0];
+      wide = wide && tile_h == 32;
       b.variant = fr.raw.format | (malvar ? 4 : 0) | (sharpen ? 8 : 0) | (tile_h == 16 ? 16 : 0);
       b.tiles_x = ceil_div(fr.raw.width, kBatchTileW);
       b.tiles = b.tiles_x * ceil_div(fr.raw.height, tile_h);
@@ -218,7 +221,7 @@ struct isp_batch {
     const int main_blocks = main_grid();
     const auto enqueue = [&](hipStream_t st) {
       if (awb_blocks > 0) launch_batch_awb(d_table, d_control, awb_blocks, st);
-      launch_batch_main(d_table, d_control, main_blocks, sharpen, st);
+      launch_batch_main(d_table, d_control, main_blocks, sharpen, wide, st);
     };
     if (schedule.graph != 0) {
       if (exec == nullptr || graph_grid != main_blocks * 65536 + awb_blocks) {
@@ -508,7 +511,7 @@ isp_status isp_batch_tune(isp_batch* batch, isp_stream_t stream, const char* cac
         consider(c);
       }
       {
-        const int fit = batch_main_blocks_per_cu(batch->sharpen);
+        const int fit = batch_main_blocks_per_cu(batch->sharpen, batch->wide);
         for (int bp : {1, 2}) {
           if (bp >= fit || bp == best.blocks_per_cu) continue;
           isp_batch_schedule c = best;

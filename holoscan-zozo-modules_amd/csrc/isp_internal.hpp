@@ -195,8 +195,10 @@ void launch_main(const FrameDesc& f, DeviceScratch* scratch, bool malvar, bool s
 // the main-pass kernel for batches with sharpened frames differs from the one
 // without.
 int batch_awb_blocks_per_cu();
-int batch_main_blocks_per_cu(bool sharpen);
+// wide: the 512-thread kernels, for batches whose frames all run 32-row tiles.
+int batch_main_blocks_per_cu(bool sharpen, bool wide);
 void launch_batch_awb(const BatchTable* table, BatchControl* control, int grid, void* stream);
-void launch_batch_main(const BatchTable* table, BatchControl* control, int grid, bool sharpen, void* stream);
+void launch_batch_main(const BatchTable* table, BatchControl* control, int grid, bool sharpen, bool wide,
+                       void* stream);
 
 }  // namespace isp_amd

@@ -820,7 +820,6 @@ struct MainCtx {
   int n1;
   int bins;
   int copies;  // interleaved histogram copies: 4 for 256 bins
-  int copy;
   bool lsc;
   bool lut_lds;
   bool want_hist;
@@ -836,7 +835,6 @@ __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains
   c.want_stats = f.stats != nullptr || f.ae_state != nullptr;
   c.bins = f.bins;
   c.copies = kMaxHistBins / c.bins;
-  c.copy = (tid & 63) & (c.copies - 1);
 #pragma unroll
   for (int k = 0; k < 3; ++k) c.gains[k] = gains[k];
   c.qc = quad_consts(f, gains);
@@ -870,26 +868,26 @@ struct PreRow {
 // preprocess (isp_math.hpp:151-155) of one row chunk into sample-tile row
 // `out`: (raw - black) / range, cut at 0, lens shading, white balance, cut at 1;
 // saturated samples of the tile's own rows and columns counted. With the table,
-// the chunk's kC entries come first in 128-bit LDS reads, so that no sample
-// waits on its own read.
+// each group of 4 samples reads its entries in two 128-bit LDS loads, in
+// straight-line code, so the loads are issued ahead of the arithmetic.
 template <int kFmt, int kLsc, int kPad, int kSW>
 __device__ __forceinline__ void pre_chunk(const FrameDesc& f, const PreRow& r, const int (&raw)[Chunk<kFmt>::kSamples],
                                           const F2* table, float tyr, float* out, unsigned int& saturated) {
   constexpr int kC = Chunk<kFmt>::kSamples;
-  float gain[kC];
-  if constexpr (kLsc == kLscTable) {
-    const F4* e = reinterpret_cast<const F4*>(table);
-#pragma unroll
-    for (int h = 0; h < kC / 2; ++h) {
-      const F4 v = e[h];
-      gain[2 * h] = v.v[0] + v.v[1] * tyr;
-      gain[2 * h + 1] = v.v[2] + v.v[3] * tyr;
-    }
-  }
 #pragma unroll
   for (int h = 0; h < kC / 4; ++h) {
     const int col0 = kPad - kC + r.ci * kC + 4 * h;
     if (col0 < 0 || col0 + 4 > kSW) continue;
+    float gain[4];
+    if constexpr (kLsc == kLscTable) {
+      const F4* e = reinterpret_cast<const F4*>(table) + 2 * h;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const F4 v = e[q];
+        gain[2 * q] = v.v[0] + v.v[1] * tyr;
+        gain[2 * q + 1] = v.v[2] + v.v[3] * tyr;
+      }
+    }
     F4 o;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -898,7 +896,7 @@ __device__ __forceinline__ void pre_chunk(const FrameDesc& f, const PreRow& r, c
       float v = (static_cast<float>(raw[k]) - (odd ? r.b1 : r.b0)) * (odd ? r.i1 : r.i0);
       v = v > 0.0F ? v : 0.0F;
       if constexpr (kLsc == kLscTable) {
-        v = v * gain[k];
+        v = v * gain[m];
       } else if constexpr (kLsc == kLscSlow) {
         const int x = r.cx + k;
         const float g = x >= 0 && x < f.width ? lsc_gain(f, x, r.y, cfa_channel(f.cfa, (r.par << 1) | (x & 1))) : 1.0F;
@@ -939,7 +937,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
   const int H = f.height;
   const int bins = c.bins;
   const int copies = c.copies;
-  const int copy = c.copy;
+  const int copy = (tid & 63) & (copies - 1);  // this lane's histogram copy
   const int tyi = tile / tiles_x;
   const int x0 = (tile - tyi * tiles_x) * 64;
   const int y0 = tyi * kTileH;
@@ -1417,17 +1415,22 @@ constexpr std::size_t max_size() {
 // The largest tile LDS of every body a main-pass batch kernel runs: with
 // sharpening, 32-row tiles dominate (RAW10's 16-sample chunks widen its lens
 // shading table); without it the LDS is small enough for 3 blocks per CU.
-template <bool kSharpen>
+template <bool kSharpen, int kBlock>
 constexpr std::size_t batch_tile_lds() {
   if constexpr (kSharpen) {
-    return max_size<MainTileLds<ISP_RAW_U8, true, true, 32>, MainTileLds<ISP_RAW_U16, true, true, 32>,
-                    MainTileLds<ISP_RAW_RAW10, true, true, 32>, MainTileLds<ISP_RAW_RAW12, true, true, 32>,
-                    MainTileLds<ISP_RAW_RAW10, false, true, 32>, MainTileLds<ISP_RAW_RAW12, false, true, 32>,
-                    MainTileLds<ISP_RAW_RAW10, true, false, 32>>();
+    return max_size<MainTileLds<ISP_RAW_U8, true, true, 32, kBlock>, MainTileLds<ISP_RAW_U16, true, true, 32, kBlock>,
+                    MainTileLds<ISP_RAW_RAW10, true, true, 32, kBlock>,
+                    MainTileLds<ISP_RAW_RAW12, true, true, 32, kBlock>,
+                    MainTileLds<ISP_RAW_RAW10, false, true, 32, kBlock>,
+                    MainTileLds<ISP_RAW_RAW12, false, true, 32, kBlock>,
+                    MainTileLds<ISP_RAW_RAW10, true, false, 32, kBlock>>();
   } else {
-    return max_size<MainTileLds<ISP_RAW_U8, true, false, 32>, MainTileLds<ISP_RAW_U16, true, false, 32>,
-                    MainTileLds<ISP_RAW_RAW10, true, false, 32>, MainTileLds<ISP_RAW_RAW12, true, false, 32>,
-                    MainTileLds<ISP_RAW_RAW10, false, false, 32>, MainTileLds<ISP_RAW_RAW12, false, false, 32>>();
+    return max_size<MainTileLds<ISP_RAW_U8, true, false, 32, kBlock>,
+                    MainTileLds<ISP_RAW_U16, true, false, 32, kBlock>,
+                    MainTileLds<ISP_RAW_RAW10, true, false, 32, kBlock>,
+                    MainTileLds<ISP_RAW_RAW12, true, false, 32, kBlock>,
+                    MainTileLds<ISP_RAW_RAW10, false, false, 32, kBlock>,
+                    MainTileLds<ISP_RAW_RAW12, false, false, 32, kBlock>>();
   }
 }
 
@@ -1435,14 +1438,15 @@ constexpr int variant_of(int fmt, bool malvar, bool sharpen, int tile_h) {
   return fmt | (malvar ? 4 : 0) | (sharpen ? 8 : 0) | (tile_h == 16 ? 16 : 0);
 }
 
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH>
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock>
 __device__ __forceinline__ void batch_main_item(const BatchFrame& b, const MainCtx& c, MainFrameLds& fl,
                                                 unsigned char* tile_lds, int item, unsigned long long acc[4]) {
-  auto& tl = *reinterpret_cast<MainTileLds<kFmt, kMalvar, kSharpen, kTileH>*>(tile_lds);
+  auto& tl = *reinterpret_cast<MainTileLds<kFmt, kMalvar, kSharpen, kTileH, kBlock>*>(tile_lds);
   const int first = item * b.tiles_per_item;
   const int end = min(first + b.tiles_per_item, b.tiles);
   for (int t = first; t < end; ++t) {
-    main_tile<kFmt, kMalvar, kSharpen, kTileH>(b.f, c, fl, tl, t, b.tiles_x, b.fast_rows, b.out_aligned, acc);
+    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock>(b.f, c, fl, tl, t, b.tiles_x, b.fast_rows, b.out_aligned,
+                                                       acc);
   }
 }
 
@@ -1593,9 +1597,11 @@ __global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* _
   q.leave();
 }
 
-#define ISP_BATCH_MAIN_CASE(FMT, MAL, SHP, TH)                                                            \
-  case variant_of(FMT, MAL, SHP, TH):                                                                     \
-    if constexpr (kSharpen || !SHP) batch_main_item<FMT, MAL, SHP, TH>(b, mc, fl, tile_lds, local, acc); \
+#define ISP_BATCH_MAIN_CASE(FMT, MAL, SHP, TH)                                        \
+  case variant_of(FMT, MAL, SHP, TH):                                                 \
+    if constexpr ((kSharpen || !SHP) && (kBlock == kThreads || TH == 32)) {           \
+      batch_main_item<FMT, MAL, SHP, TH, kBlock>(b, mc, fl, tile_lds, local, acc);    \
+    }                                                                                 \
     break;
 #define ISP_BATCH_MAIN_CASES(FMT)            \
   ISP_BATCH_MAIN_CASE(FMT, false, false, 32) \
@@ -1609,23 +1615,28 @@ __global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* _
 
 // The main pass of every frame of a batch in one persistent grid. kSharpen: the
 // batch has frames with sharpening (their tile bodies need the larger LDS); a
-// batch without them runs a lighter kernel at 3 blocks per CU.
-template <bool kSharpen>
+// batch without them runs a lighter kernel. kBlock: 512-thread blocks (16 waves
+// per CU, as the per-frame kernel) run 32-row tiles only; a batch with 16-row
+// tiles runs the 256-thread kernels.
+template <bool kSharpen, int kBlock>
 __device__ __forceinline__ void batch_main_body(const BatchTable* __restrict__ table, BatchControl* ctl) {
   __shared__ MainFrameLds fl;
-  __shared__ __attribute__((aligned(16))) unsigned char tile_lds[batch_tile_lds<kSharpen>()];
+  __shared__ __attribute__((aligned(16))) unsigned char tile_lds[batch_tile_lds<kSharpen, kBlock>()];
   __shared__ int slot;
   BatchQueue q{&ctl->main_next, &ctl->main_exited, table->main_items, table->queue, &slot};
   const int count = table->count;
   int cur = -1;
   unsigned int items_done = 0;
   unsigned long long acc[4] = {0, 0, 0, 0};
-  MainCtx mc;
+  // The frame's constants stay in LDS, read where the tile bodies use them:
+  // held in registers across the switch over all variants they would crowd
+  // out the tile bodies' own.
+  __shared__ MainCtx mc;
   auto close = [&]() {
     const BatchFrame& pb = table->frames[cur];
-    const unsigned int token = main_flush(&pb.s->main, fl, mc, acc);
+    const unsigned int token = main_flush<kBlock>(&pb.s->main, fl, mc, acc);
     if (complete_items(&pb.s->main.done[0].n, items_done, static_cast<unsigned int>(pb.items), token, &fl.flag)) {
-      main_finish(pb.f, &pb.s->main, fl, mc);
+      main_finish<kBlock>(pb.f, &pb.s->main, fl, mc);
       if (threadIdx.x == 0) atomicExch(&pb.s->main.done[0].n, 0U);
     }
   };
@@ -1647,7 +1658,10 @@ __device__ __forceinline__ void batch_main_body(const BatchTable* __restrict__ t
       float gains[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) gains[k] = b.f.wb_mode != ISP_WB_MANUAL ? b.s->wb.gains[k] : b.f.wb[k];
-      main_begin(b.f, gains, fl, mc);
+      MainCtx begun;
+      main_begin<kBlock>(b.f, gains, fl, begun);
+      if (threadIdx.x == 0) mc = begun;
+      __syncthreads();
     }
     switch (b.variant) {
       ISP_BATCH_MAIN_CASES(ISP_RAW_U8)
@@ -1668,14 +1682,25 @@ __device__ __forceinline__ void batch_main_body(const BatchTable* __restrict__ t
 
 __global__ void __launch_bounds__(kThreads) batch_main_kernel_sharpen(const BatchTable* __restrict__ table,
                                                                       BatchControl* ctl) {
-  batch_main_body<true>(table, ctl);
+  batch_main_body<true, kThreads>(table, ctl);
 }
 
 // Three resident blocks per CU: the registers are held to what three waves per
 // SIMD leave (the LDS of this kernel fits three blocks).
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 3)))
 batch_main_kernel_plain(const BatchTable* __restrict__ table, BatchControl* ctl) {
-  batch_main_body<false>(table, ctl);
+  batch_main_body<false, kThreads>(table, ctl);
+}
+
+// 512-thread forms, registers held to what four waves per SIMD leave.
+constexpr int kWideBlock = 512;
+__global__ void __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(4)))
+batch_main_kernel_sharpen_wide(const BatchTable* __restrict__ table, BatchControl* ctl) {
+  batch_main_body<true, kWideBlock>(table, ctl);
+}
+__global__ void __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(4)))
+batch_main_kernel_plain_wide(const BatchTable* __restrict__ table, BatchControl* ctl) {
+  batch_main_body<false, kWideBlock>(table, ctl);
 }
 
 // ---- launches ---------------------------------------------------------------
@@ -1792,7 +1817,11 @@ void launch_main(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen
   }
 }
 
-int batch_main_blocks_per_cu(bool sharpen) {
+int batch_main_blocks_per_cu(bool sharpen, bool wide) {
+  if (wide) {
+    return sharpen ? blocks_per_cu(batch_main_kernel_sharpen_wide, kWideBlock)
+                   : blocks_per_cu(batch_main_kernel_plain_wide, kWideBlock);
+  }
   return sharpen ? blocks_per_cu(batch_main_kernel_sharpen) : blocks_per_cu(batch_main_kernel_plain);
 }
 
@@ -1804,9 +1833,16 @@ void launch_batch_awb(const BatchTable* table, BatchControl* control, int grid, 
   check(hipGetLastError(), "batch_awb_kernel launch");
 }
 
-void launch_batch_main(const BatchTable* table, BatchControl* control, int grid, bool sharpen, void* stream) {
+void launch_batch_main(const BatchTable* table, BatchControl* control, int grid, bool sharpen, bool wide,
+                       void* stream) {
   const hipStream_t st = static_cast<hipStream_t>(stream);
-  if (sharpen) {
+  if (wide) {
+    if (sharpen) {
+      hipLaunchKernelGGL(batch_main_kernel_sharpen_wide, dim3(grid), dim3(kWideBlock), 0, st, table, control);
+    } else {
+      hipLaunchKernelGGL(batch_main_kernel_plain_wide, dim3(grid), dim3(kWideBlock), 0, st, table, control);
+    }
+  } else if (sharpen) {
     hipLaunchKernelGGL(batch_main_kernel_sharpen, dim3(grid), dim3(kThreads), 0, st, table, control);
   } else {
     hipLaunchKernelGGL(batch_main_kernel_plain, dim3(grid), dim3(kThreads), 0, st, table, control);
