@@ -82,6 +82,30 @@ def traffic_from_counters():
     return round(copy_scaled), detail
 
 
+VALU_PEAK_TLANE_OPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, unpacked fp32
+
+
+def valu_from_counters(main_ms, pixels):
+    """The main kernel against its arithmetic ceiling: lane-operations per launch from the
+    committed rocprofv3 SQ_INSTS_VALU reading (wave instructions x 64 lanes) over the live
+    average launch time, against 256 CUs x 64 lanes x 2.4 GHz; and the VALU-busy share of the
+    profiled run (SQ_ACTIVE_INST_VALU x 4 cycles over 4 SIMDs x 256 CUs x the kernel's cycles
+    per XCD)."""
+    path = os.path.join(ROOT, "profiles", "main_kernel_4k_raw12_malvar_counters.json")
+    if not os.path.exists(path) or main_ms <= 0:
+        return None
+    with open(path) as fh:
+        c = json.load(fh)
+    lane_ops = c["sq_insts_valu_per_dispatch"] * 64.0
+    achieved = lane_ops / (main_ms * 1e-3) / 1e12
+    busy = c["sq_active_inst_valu_per_dispatch"] * 4.0 / (
+        4 * 256 * c["grbm_gui_active_cycles_per_dispatch"] / c["xcds"])
+    return {"bound": "valu", "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TLANE_OPS, 2),
+            "unit": "T lane-ops/s", "frac": round(achieved / VALU_PEAK_TLANE_OPS, 4),
+            "lane_ops_per_px": round(lane_ops / pixels, 1), "busy_profiled": round(busy, 3),
+            "source": os.path.relpath(path, ROOT)}
+
+
 def percentile(values, q):
     s = sorted(values)
     if not s:
@@ -528,7 +552,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "main_kernel<malvar,sharpen>", "algorithmic_bytes_per_launch": algo_bytes,
-                     "traffic_detail": traffic_detail},
+                     "traffic_detail": traffic_detail,
+                     "valu": valu_from_counters(main_ms / n, pixels)},
     }
     if args.batch_ticks > 0:
         result["batched_16_streams"] = batched_vs_per_frame(args.batch_ticks)
