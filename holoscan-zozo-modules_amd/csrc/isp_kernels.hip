@@ -614,6 +614,28 @@ __device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl
   return token;
 }
 
+// Gray-world levels from the sums (finish_white_balance, isp_math.hpp:366-370):
+// the means in double, G over both green samples of a quad.
+__device__ __forceinline__ void gray_world_levels(const unsigned long long sums[4], float level[3]) {
+  const double quads = static_cast<double>(sums[3]);
+  level[0] = static_cast<float>(static_cast<double>(sums[0]) / quads);
+  level[1] = static_cast<float>(static_cast<double>(sums[1]) / (2.0 * quads));
+  level[2] = static_cast<float>(static_cast<double>(sums[2]) / quads);
+}
+
+// The rest of finish_white_balance (isp_math.hpp:371-384): the clamped target
+// gains, smoothed from the state `old` unless it is unset or smoothing is 1.
+// False (state kept) when a level is not positive.
+__device__ __forceinline__ bool wb_target(const float level[3], const float old_gains[3], int old_valid,
+                                          float smoothing, float out[3]) {
+  if (!(level[0] > 0.0F && level[1] > 0.0F && level[2] > 0.0F)) return false;
+  float target[3] = {level[1] / level[0], 1.0F, level[1] / level[2]};
+  for (int k = 0; k < 3; ++k) target[k] = target[k] < 0.0625F ? 0.0625F : target[k] > 16.0F ? 16.0F : target[k];
+  const bool smooth = old_valid != 0 && smoothing < 1.0F;
+  for (int k = 0; k < 3; ++k) out[k] = smooth ? old_gains[k] + smoothing * (target[k] - old_gains[k]) : target[k];
+  return true;
+}
+
 // The frame's last contributor: finish_white_balance (isp_math.hpp:357-384)
 // into the stream's white balance state; the reads clear the scratch.
 __device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s, AwbFrameLds& fl, const AwbCtx& c) {
@@ -635,10 +657,8 @@ __device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s,
         }
       }
     } else if (tid == 0) {
-      const double quads = static_cast<double>(quads_s);
-      fl.level[0] = static_cast<float>(static_cast<double>(fl.sums[0]) / quads);
-      fl.level[1] = static_cast<float>(static_cast<double>(fl.sums[1]) / (2.0 * quads));
-      fl.level[2] = static_cast<float>(static_cast<double>(fl.sums[2]) / quads);
+      const unsigned long long sums[4] = {fl.sums[0], fl.sums[1], fl.sums[2], fl.sums[3]};
+      gray_world_levels(sums, fl.level);
     }
   }
   __syncthreads();
@@ -646,18 +666,12 @@ __device__ __forceinline__ void awb_finish(const FrameDesc& f, DeviceScratch* s,
     unsigned int token = 0;
     if (quads_s != 0) {
       const float level[3] = {fl.level[0], fl.level[1], fl.level[2]};
-      if (level[0] > 0.0F && level[1] > 0.0F && level[2] > 0.0F) {
-        float target[3] = {level[1] / level[0], 1.0F, level[1] / level[2]};
-        for (int k = 0; k < 3; ++k) {
-          target[k] = target[k] < 0.0625F ? 0.0625F : target[k] > 16.0F ? 16.0F : target[k];
-        }
-        // The state was last written by an earlier launch.
-        WbState& st = s->wb;
-        const bool smooth = st.valid != 0 && f.smoothing < 1.0F;
-        for (int k = 0; k < 3; ++k) {
-          const float g = smooth ? st.gains[k] + f.smoothing * (target[k] - st.gains[k]) : target[k];
-          token ^= coherent_write(&st.gains[k], g);
-        }
+      // The state was last written by an earlier launch.
+      WbState& st = s->wb;
+      const float old[3] = {st.gains[0], st.gains[1], st.gains[2]};
+      float g[3];
+      if (wb_target(level, old, st.valid, f.smoothing, g)) {
+        for (int k = 0; k < 3; ++k) token ^= coherent_write(&st.gains[k], g[k]);
         token ^= atomicExch(reinterpret_cast<unsigned int*>(&st.valid), 1U);
       }
     }
@@ -675,6 +689,9 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
   unsigned long long local[kAwbAcc] = {0, 0, 0, 0, 0};
   awb_tiles<kFmt>(f, c, fl, tl, blockIdx.x, tiles, gridDim.x, tiles_x, fast_rows, local);
   const unsigned int token = awb_flush(&s->awb, fl, c, local);
+  // Gray world: the sums stay in the scratch, and the main pass that follows on
+  // the stream turns them into gains (main_gray_world_gains); no completion here.
+  if (!c.patch) return;
   if (!complete_grid(s->awb.done, &s->awb.top, token, &fl.flag)) return;
   awb_finish(f, s, fl, c);
   reset_grid(s->awb.done, &s->awb.top);
@@ -801,6 +818,7 @@ struct MainFrameLds {
   unsigned long long part[8][4];  // per wave, up to 512 threads
   unsigned long long sums[4];
   float gains[3];
+  int wb_update;  // gray world: the frame's gains replace the white balance state
   int flag;
 };
 template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
@@ -826,6 +844,53 @@ struct MainCtx {
   bool want_stats;
 };
 
+__device__ __forceinline__ void main_set_gains(const FrameDesc& f, const float gains[3], MainCtx& c) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c.gains[k] = gains[k];
+  c.qc = quad_consts(f, gains);
+}
+
+// Gray world in the per-frame form: the white balance kernel leaves its sums in
+// the scratch, and each block of the main pass turns them into the frame's
+// gains (finish_white_balance, isp_math.hpp:357-384) with the state an earlier
+// frame left, all blocks alike; the last block stores the state and clears the
+// sums (main_gray_world_store). No block changes either before every block has
+// read them: the last block is known only once all have counted. Thread 0
+// reads and computes while the other threads stage the block's tables; the
+// result is read after a barrier.
+__device__ __forceinline__ void main_gray_world_gains(const FrameDesc& f, const DeviceScratch* s, MainFrameLds& fl) {
+  if (threadIdx.x != 0) return;
+  unsigned long long sums[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int x = 0; x < kSlots; ++x) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sums[k] += s->awb.sum[x].v[k];
+  }
+  const float old[3] = {s->wb.gains[0], s->wb.gains[1], s->wb.gains[2]};
+  float g[3] = {old[0], old[1], old[2]};
+  int update = 0;
+  if (sums[3] != 0) {
+    float level[3];
+    gray_world_levels(sums, level);
+    update = wb_target(level, old, s->wb.valid, f.smoothing, g) ? 1 : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) fl.gains[k] = g[k];
+  fl.wb_update = update;
+}
+
+// The last block of a gray-world main pass: the state for the next frame, and
+// the white balance sums cleared.
+__device__ __forceinline__ void main_gray_world_store(DeviceScratch* s, const MainFrameLds& fl) {
+  const int tid = threadIdx.x;
+  if (tid < 4 * kSlots) atomicExch(&s->awb.sum[tid >> 2].v[tid & 3], 0ULL);
+  if (tid == 0 && fl.wb_update != 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) coherent_write(&s->wb.gains[k], fl.gains[k]);
+    atomicExch(reinterpret_cast<unsigned int*>(&s->wb.valid), 1U);
+  }
+}
+
 // Starts a block's work on a frame. The tile's first barrier orders these LDS
 // writes before any reader.
 template <int kBlock = kThreads>
@@ -835,9 +900,7 @@ __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains
   c.want_stats = f.stats != nullptr || f.ae_state != nullptr;
   c.bins = f.bins;
   c.copies = kMaxHistBins / c.bins;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) c.gains[k] = gains[k];
-  c.qc = quad_consts(f, gains);
+  main_set_gains(f, gains, c);
   if (c.want_hist) {
     for (int i = tid; i < 4 * kMaxHistBins; i += kBlock) fl.hist[i] = 0U;
   }
@@ -1381,16 +1444,23 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBl
 main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows, int out_aligned) {
   __shared__ MainFrameLds fl;
   __shared__ MainTileLds<kFmt, kMalvar, kSharpen, kTileH, kBlock> tl;
+  const bool gray_world = f.wb_mode == ISP_WB_GRAY_WORLD;
   float gains[3];
-  if (f.wb_mode != ISP_WB_MANUAL) {
+  if (f.wb_mode != ISP_WB_MANUAL && !gray_world) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) gains[k] = s->wb.gains[k];
   } else {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) gains[k] = f.wb[k];
+    for (int k = 0; k < 3; ++k) gains[k] = f.wb[k];  // gray world: replaced below
   }
+  if (gray_world) main_gray_world_gains(f, s, fl);
   MainCtx c;
   main_begin<kBlock>(f, gains, fl, c);
+  if (gray_world) {
+    __syncthreads();
+    const float g[3] = {fl.gains[0], fl.gains[1], fl.gains[2]};
+    main_set_gains(f, g, c);
+  }
   unsigned long long acc[4] = {0, 0, 0, 0};
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock>(f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
@@ -1398,6 +1468,7 @@ main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows
   const unsigned int token = main_flush<kBlock>(&s->main, fl, c, acc);
   if (!complete_grid(s->main.done, &s->main.top, token, &fl.flag)) return;
   main_finish<kBlock>(f, &s->main, fl, c);
+  if (gray_world) main_gray_world_store(s, fl);
   reset_grid(s->main.done, &s->main.top);
 }
 
