@@ -1,14 +1,18 @@
 // isp_color.hip - colour conversion of the ISP's RGB output to the YCbCr
 // layouts encoders take (include/isp_color.h).
 //
-// The conversion is HBM-bound (RGB8 to NV12 reads 3 and writes 1.5 bytes a
-// pixel), so the kernel is shaped for the memory system: one thread converts a
-// span of 8 pixels (two rows of them for the 4:2:0 layouts, whose chroma sample
-// covers a 2 x 2 block), loading its rows as 8- or 16-byte vectors and storing
-// each plane's bytes of the span as one vector, consecutive threads on
-// consecutive spans. Spans at the right or bottom edge, and views whose planes
-// or pitches are not aligned for the vectors, take a byte-wise path with the
-// same arithmetic.
+// The conversion moves few bytes a pixel (RGB8 to NV12 reads 3 and writes 1.5)
+// but spends about 30 float operations on each (nine products, nine sums, the
+// rounding of a luma and a share of two chroma codes), so at 4K it is bound by
+// the vector ALUs rather than by HBM, and the kernel is written to spend as few
+// operations as the reference's arithmetic allows: one thread converts a span
+// of 8 pixels (two rows of them for the 4:2:0 layouts, whose chroma sample
+// covers a 2 x 2 block), converts bytes with v_cvt_f32_ubyte*, clamps with one
+// median and packs each code into its output word with v_cvt_pk_u8_f32. Rows
+// are loaded and planes stored as the widest vectors their alignment allows,
+// consecutive threads on consecutive spans. Spans at the right or bottom edge,
+// and views whose planes or pitches are not aligned for the vectors, take a
+// byte-wise path with the same arithmetic.
 //
 // Arithmetic, as the reference's per-pixel code (kernel_math.hpp:80-361) does
 // it, in float without contraction: a pixel's (Y, Cb, Cr) is m * (R, G, B) + o
@@ -178,10 +182,25 @@ struct Ycc {
   float y, cb, cr;
 };
 
-// to_u8 (kernel_math.hpp:80-83).
+// to_u8 (kernel_math.hpp:80-83) up to the cast: floor(v + 0.5) clamped to
+// 0..255, an exact integer in float. The values here are finite, so the clamp
+// is one median of three.
+__device__ __forceinline__ float code_f(float v) {
+  return __builtin_amdgcn_fmed3f(floorf(v + 0.5F), 0.0F, 255.0F);
+}
 __device__ __forceinline__ unsigned int code(float v) {
-  const float r = floorf(v + 0.5F);
-  return static_cast<unsigned int>(static_cast<std::uint8_t>(r <= 0.0F ? 0.0F : (r >= 255.0F ? 255.0F : r)));
+  return static_cast<unsigned int>(code_f(v));
+}
+// Puts code_f(v) into byte `i` of `word` (one v_cvt_pk_u8_f32: the value is an
+// integer in 0..255, so the conversion is exact).
+__device__ __forceinline__ unsigned int put_code(float v, unsigned int i, unsigned int word) {
+  return __builtin_amdgcn_cvt_pk_u8_f32(code_f(v), i, word);
+}
+// put_code for a luma value, which needs no clamp: its weights are positive
+// and sum to at most 1 (to 219/255 over a 16 offset in limited range), so v
+// lies in [0, 255] up to a few ulps and floor(v + 0.5) in 0..255.
+__device__ __forceinline__ unsigned int put_luma_code(float v, unsigned int i, unsigned int word) {
+  return __builtin_amdgcn_cvt_pk_u8_f32(floorf(v + 0.5F), i, word);
 }
 
 // apply (kernel_math.hpp:234-242) to the source's bytes.
@@ -220,7 +239,15 @@ __device__ __forceinline__ Ycc pixel_of(const Affine& a, const unsigned int* w, 
   return apply(a, c0, c1, c2);
 }
 
-constexpr int kSpan = 8;  // pixels of a thread's span
+// Pixels of a thread's span. 8 measures faster at 4K than 16 (more waves,
+// fewer registers), and faster than 8 with several span rows a thread.
+constexpr int kSpan = 8;
+
+// The widest vector (4, 8 or 16 bytes) that a run of `bytes` bytes starting at
+// a multiple of `bytes` keeps aligned.
+constexpr int unit_of(int bytes) {
+  return bytes % 16 == 0 ? 16 : (bytes % 8 == 0 ? 8 : 4);
+}
 
 template <int kDst>
 constexpr int block_rows() {
@@ -293,110 +320,178 @@ __device__ void span_bytes(const Affine& a, const std::uint8_t* src, std::size_t
   }
 }
 
-// One span row's pixels from its vector loads.
+// Words of one span row of the source (kSpan pixels).
 template <int kSrc>
-__device__ __forceinline__ void load_span(const std::uint8_t* p, unsigned int (&w)[2 * src_bytes<kSrc>()]) {
-  if constexpr (src_bytes<kSrc>() == 3) {
+constexpr int span_words() {
+  return kSpan * src_bytes<kSrc>() / 4;
+}
+
+typedef unsigned int U4 __attribute__((ext_vector_type(4)));
+typedef unsigned int U2 __attribute__((ext_vector_type(2)));
+
+// kN words from p, as vectors of kUnit bytes (p aligned to kUnit).
+template <int kUnit, int kN>
+__device__ __forceinline__ void load_words(const std::uint8_t* p, unsigned int (&w)[kN]) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const V2 v = reinterpret_cast<const V2*>(p)[i];
-      w[2 * i] = v.w[0];
-      w[2 * i + 1] = v.w[1];
-    }
-  } else {
+  for (int i = 0; i < kN; i += kUnit / 4) {
+    if constexpr (kUnit == 16) {
+      const V4 v = *reinterpret_cast<const V4*>(p + 4 * i);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const V4 v = reinterpret_cast<const V4*>(p)[i];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) w[4 * i + k] = v.w[k];
+      for (int k = 0; k < 4; ++k) w[i + k] = v.w[k];
+    } else if constexpr (kUnit == 8) {
+      const V2 v = *reinterpret_cast<const V2*>(p + 4 * i);
+      w[i] = v.w[0];
+      w[i + 1] = v.w[1];
+    } else {
+      w[i] = *reinterpret_cast<const unsigned int*>(p + 4 * i);
     }
   }
 }
 
-__device__ __forceinline__ unsigned int pack4(const unsigned int* b) {
-  return b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+// kN words to p, as vectors of kUnit bytes (p aligned to kUnit). The stores
+// are nontemporal: the planes are written once and read by the next stage,
+// not by this kernel, and streaming them past the caches measures about 10 %
+// faster at 4K.
+template <int kUnit, int kN>
+__device__ __forceinline__ void store_words(std::uint8_t* p, const unsigned int (&w)[kN]) {
+#pragma unroll
+  for (int i = 0; i < kN; i += kUnit / 4) {
+    if constexpr (kUnit == 16) {
+      __builtin_nontemporal_store(U4{w[i], w[i + 1], w[i + 2], w[i + 3]}, reinterpret_cast<U4*>(p + 4 * i));
+    } else if constexpr (kUnit == 8) {
+      __builtin_nontemporal_store(U2{w[i], w[i + 1]}, reinterpret_cast<U2*>(p + 4 * i));
+    } else {
+      __builtin_nontemporal_store(w[i], reinterpret_cast<unsigned int*>(p + 4 * i));
+    }
+  }
 }
 
-// Grid: x over spans of 8 pixels, y over span rows (of block_rows() image rows).
+// Vector units of the source rows, plane 0 and the chroma planes.
+template <int kSrc>
+constexpr int src_unit() {
+  return unit_of(kSpan * src_bytes<kSrc>());
+}
+template <int kDst>
+constexpr int luma_unit() {
+  return unit_of(kDst == ISP_PIX_YUYV || kDst == ISP_PIX_UYVY ? 2 * kSpan : kSpan);
+}
+template <int kDst>
+constexpr int chroma_unit() {
+  return unit_of(kDst == ISP_PIX_I420 ? kSpan / 2 : kSpan);
+}
+
+
+// One span (kSpan pixels of kBY rows at (x0, y0)) from its loaded words to
+// every destination plane, with vector stores.
+template <int kSrc, int kDst>
+__device__ __forceinline__ void convert_span(const Affine& a, const unsigned int (&w)[block_rows<kDst>()][span_words<kSrc>()],
+                                             const Planes& d, int x0, int y0) {
+  constexpr int kBY = block_rows<kDst>();
+  constexpr int kBlocks = kSpan / 2;  // chroma blocks of a span
+  // Row by row: each pixel's luma code, put in its byte of the output word (a
+  // plane-0 word of 4 codes; for packed 4:2:2 the word of its pair), and its Cb
+  // and Cr added to its chroma block's sums, which so run row by row from 0 as
+  // the reference's do.
+  constexpr bool kPacked = kDst == ISP_PIX_YUYV || kDst == ISP_PIX_UYVY;
+  unsigned int y[kBY][kPacked ? kBlocks : kSpan / 4];
+  float sb[kBlocks];
+  float sr[kBlocks];
+#pragma unroll
+  for (int r = 0; r < kBY; ++r) {
+#pragma unroll
+    for (int k = 0; k < kSpan; ++k) {
+      const Ycc v = pixel_of<kSrc>(a, w[r], k);
+      const int word = kPacked ? k >> 1 : k >> 2;
+      const int byte = kPacked ? (kDst == ISP_PIX_UYVY ? 1 : 0) + 2 * (k & 1) : k & 3;
+      const bool first = kPacked ? (k & 1) == 0 : (k & 3) == 0;
+      y[r][word] = put_luma_code(v.y, byte, first ? 0U : y[r][word]);
+      if constexpr (kDst != ISP_PIX_MONO8) {
+        if (r == 0 && (k & 1) == 0) {
+          sb[k >> 1] = v.cb;
+          sr[k >> 1] = v.cr;
+        } else {
+          sb[k >> 1] += v.cb;
+          sr[k >> 1] += v.cr;
+        }
+      }
+    }
+  }
+  if constexpr (kDst == ISP_PIX_MONO8) {
+    store_words<luma_unit<kDst>()>(d.data[0] + static_cast<std::size_t>(y0) * d.pitch[0] + x0, y[0]);
+  } else {
+    // Chroma means: the division by a power of two is exact as a product.
+    float cb[kBlocks];
+    float cr[kBlocks];
+    const float count = static_cast<float>(2 * kBY);
+#pragma unroll
+    for (int b = 0; b < kBlocks; ++b) {
+      cb[b] = sb[b] / count;
+      cr[b] = sr[b] / count;
+    }
+    if constexpr (kDst == ISP_PIX_YUYV || kDst == ISP_PIX_UYVY) {
+      unsigned int o[kBlocks];
+#pragma unroll
+      for (int b = 0; b < kBlocks; ++b) {
+        o[b] = kDst == ISP_PIX_YUYV ? put_code(cr[b], 3, put_code(cb[b], 1, y[0][b]))
+                                    : put_code(cr[b], 2, put_code(cb[b], 0, y[0][b]));
+      }
+      store_words<luma_unit<kDst>()>(d.data[0] + static_cast<std::size_t>(y0) * d.pitch[0] + 2 * x0, o);
+    } else {
+#pragma unroll
+      for (int r = 0; r < kBY; ++r) store_words<luma_unit<kDst>()>(d.data[0] + static_cast<std::size_t>(y0 + r) * d.pitch[0] + x0, y[r]);
+      const int cy = y0 / kBY;
+      if constexpr (kDst == ISP_PIX_I420) {
+        unsigned int u[kBlocks / 4];
+        unsigned int v[kBlocks / 4];
+#pragma unroll
+        for (int i = 0; i < kBlocks / 4; ++i) {
+          u[i] = 0;
+          v[i] = 0;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            u[i] = put_code(cb[4 * i + k], k, u[i]);
+            v[i] = put_code(cr[4 * i + k], k, v[i]);
+          }
+        }
+        store_words<chroma_unit<kDst>()>(d.data[1] + static_cast<std::size_t>(cy) * d.pitch[1] + (x0 >> 1), u);
+        store_words<chroma_unit<kDst>()>(d.data[2] + static_cast<std::size_t>(cy) * d.pitch[2] + (x0 >> 1), v);
+      } else {
+        unsigned int o[kBlocks / 2];
+#pragma unroll
+        for (int i = 0; i < kBlocks / 2; ++i) {
+          o[i] = 0;
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            o[i] = put_code(cb[2 * i + k], 2 * k, o[i]);
+            o[i] = put_code(cr[2 * i + k], 2 * k + 1, o[i]);
+          }
+        }
+        store_words<chroma_unit<kDst>()>(d.data[1] + static_cast<std::size_t>(cy) * d.pitch[1] + x0, o);
+      }
+    }
+  }
+}
+
+// Grid: x over spans of kSpan pixels, y over span rows (of block_rows() image rows).
 template <int kSrc, int kDst>
 __global__ void __launch_bounds__(256) rgb_to_ycc_kernel(const std::uint8_t* __restrict__ src, std::size_t src_pitch,
                                                          Planes d, Affine a, int W, int H, int vec) {
   constexpr int kBY = block_rows<kDst>();
-  constexpr int kWords = 2 * src_bytes<kSrc>();
   const int x0 = kSpan * static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  const int y0 = kBY * static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+  // A wave is one row of 64 threads (the block is 64 x 4), so its span row is
+  // uniform: read from the first lane, its row addresses are scalar.
+  const int y0 = __builtin_amdgcn_readfirstlane(kBY * static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y));
   if (x0 >= W || y0 >= H) return;
   if (vec == 0 || x0 + kSpan > W || y0 + kBY > H) {
     span_bytes<kSrc, kDst>(a, src, src_pitch, d, W, H, x0, y0);
     return;
   }
-  unsigned int w[kBY][kWords];
+  unsigned int w[kBY][span_words<kSrc>()];
 #pragma unroll
   for (int r = 0; r < kBY; ++r) {
-    load_span<kSrc>(src + static_cast<std::size_t>(y0 + r) * src_pitch + src_bytes<kSrc>() * x0, w[r]);
+    load_words<src_unit<kSrc>()>(src + static_cast<std::size_t>(y0 + r) * src_pitch + src_bytes<kSrc>() * x0, w[r]);
   }
-  Ycc v[kBY][kSpan];
-#pragma unroll
-  for (int r = 0; r < kBY; ++r) {
-#pragma unroll
-    for (int k = 0; k < kSpan; ++k) v[r][k] = pixel_of<kSrc>(a, w[r], k);
-  }
-  unsigned int y[kBY][kSpan];
-#pragma unroll
-  for (int r = 0; r < kBY; ++r) {
-#pragma unroll
-    for (int k = 0; k < kSpan; ++k) y[r][k] = code(v[r][k].y);
-  }
-  if constexpr (kDst == ISP_PIX_MONO8) {
-    V2 o{{pack4(&y[0][0]), pack4(&y[0][4])}};
-    *reinterpret_cast<V2*>(d.data[0] + static_cast<std::size_t>(y0) * d.pitch[0] + x0) = o;
-    return;
-  } else {
-    // The span's four chroma blocks, each summed row by row from 0.
-    unsigned int cb[4];
-    unsigned int cr[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      float sb = 0.0F;
-      float sr = 0.0F;
-#pragma unroll
-      for (int r = 0; r < kBY; ++r) {
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          sb += v[r][2 * b + k].cb;
-          sr += v[r][2 * b + k].cr;
-        }
-      }
-      const float count = static_cast<float>(2 * kBY);
-      cb[b] = code(sb / count);
-      cr[b] = code(sr / count);
-    }
-    if constexpr (kDst == ISP_PIX_YUYV || kDst == ISP_PIX_UYVY) {
-      V4 o;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        o.w[b] = kDst == ISP_PIX_YUYV ? y[0][2 * b] | (cb[b] << 8) | (y[0][2 * b + 1] << 16) | (cr[b] << 24)
-                                      : cb[b] | (y[0][2 * b] << 8) | (cr[b] << 16) | (y[0][2 * b + 1] << 24);
-      }
-      *reinterpret_cast<V4*>(d.data[0] + static_cast<std::size_t>(y0) * d.pitch[0] + 2 * x0) = o;
-    } else {
-#pragma unroll
-      for (int r = 0; r < kBY; ++r) {
-        V2 o{{pack4(&y[r][0]), pack4(&y[r][4])}};
-        *reinterpret_cast<V2*>(d.data[0] + static_cast<std::size_t>(y0 + r) * d.pitch[0] + x0) = o;
-      }
-      const int cy = y0 / kBY;
-      if constexpr (kDst == ISP_PIX_I420) {
-        *reinterpret_cast<unsigned int*>(d.data[1] + static_cast<std::size_t>(cy) * d.pitch[1] + (x0 >> 1)) =
-            pack4(cb);
-        *reinterpret_cast<unsigned int*>(d.data[2] + static_cast<std::size_t>(cy) * d.pitch[2] + (x0 >> 1)) =
-            pack4(cr);
-      } else {
-        V2 o{{cb[0] | (cr[0] << 8) | (cb[1] << 16) | (cr[1] << 24), cb[2] | (cr[2] << 8) | (cb[3] << 16) | (cr[3] << 24)}};
-        *reinterpret_cast<V2*>(d.data[1] + static_cast<std::size_t>(cy) * d.pitch[1] + x0) = o;
-      }
-    }
-  }
+  convert_span<kSrc, kDst>(a, w, d, x0, y0);
 }
 
 bool aligned(const void* p, std::size_t pitch, std::size_t n) {
@@ -411,17 +506,13 @@ void launch(const isp_image_view& s, const isp_image_view& t, const Affine& a, h
     d.data[i] = t.planes[i].data;
     d.pitch[i] = t.planes[i].pitch;
   }
-  // The vector form needs the source rows at 8 (3-byte pixels) or 16 bytes, and
-  // each destination plane at the width of its store.
-  bool vec = aligned(s.planes[0].data, s.planes[0].pitch, src_bytes<kSrc>() == 3 ? 8 : 16);
-  if constexpr (kDst == ISP_PIX_YUYV || kDst == ISP_PIX_UYVY) {
-    vec = vec && aligned(d.data[0], d.pitch[0], 16);
-  } else {
-    vec = vec && aligned(d.data[0], d.pitch[0], 8);
-    if constexpr (kDst == ISP_PIX_NV12 || kDst == ISP_PIX_NV16) vec = vec && aligned(d.data[1], d.pitch[1], 8);
-    if constexpr (kDst == ISP_PIX_I420) {
-      vec = vec && aligned(d.data[1], d.pitch[1], 4) && aligned(d.data[2], d.pitch[2], 4);
-    }
+  // The vector form needs the source rows and each destination plane aligned
+  // to its vectors.
+  bool vec = aligned(s.planes[0].data, s.planes[0].pitch, src_unit<kSrc>()) &&
+             aligned(d.data[0], d.pitch[0], luma_unit<kDst>());
+  if constexpr (kDst == ISP_PIX_NV12 || kDst == ISP_PIX_NV16) vec = vec && aligned(d.data[1], d.pitch[1], chroma_unit<kDst>());
+  if constexpr (kDst == ISP_PIX_I420) {
+    vec = vec && aligned(d.data[1], d.pitch[1], chroma_unit<kDst>()) && aligned(d.data[2], d.pitch[2], chroma_unit<kDst>());
   }
   const int spans = (s.width + kSpan - 1) / kSpan;
   const int span_rows = (s.height + kBY - 1) / kBY;

@@ -32,15 +32,16 @@ def params(matrix=_abi.MATRIX_BT601, range_=_abi.RANGE_LIMITED):
 
 class Picture:
     """An image of `fmt` in one buffer, plane after plane, each row `pad` bytes longer than
-    packed: host (numpy uint8) or device (a torch uint8 tensor on cuda)."""
+    packed and then rounded up to a multiple of `align` bytes (as pitched allocators lay
+    rows out): host (numpy uint8) or device (a torch uint8 tensor on cuda)."""
 
-    def __init__(self, fmt, width, height, device=False, pad=0, fill=0):
+    def __init__(self, fmt, width, height, device=False, pad=0, fill=0, align=1):
         self.format, self.width, self.height = fmt, width, height
         self.layout = []  # (offset, pitch, row bytes, rows) per plane
         offset = 0
         for plane in range(plane_count(fmt)):
             row, rows = plane_size(fmt, width, height, plane)
-            pitch = row + pad
+            pitch = -(-(row + pad) // align) * align
             self.layout.append((offset, pitch, row, rows))
             offset += pitch * rows
         self.nbytes = offset

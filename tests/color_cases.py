@@ -13,20 +13,25 @@ RANGES = [_abi.RANGE_LIMITED, _abi.RANGE_FULL]
 PACKED_422 = (_abi.PIX_YUYV, _abi.PIX_UYVY)
 
 
+# Sizes whose rows hold whole 16-pixel spans and a ragged last one, with odd and
+# even heights, for pitches aligned to the vector loads.
+VECTOR_SIZES = [(64, 5), (72, 6), (112, 3), (200, 9), (17, 2), (48, 1), (31, 4)]
+
+
 def sizes_for(dst):
     """Sizes with odd widths and heights where the layout allows them (packed 4:2:2 needs
-    an even width), a span boundary (8 pixels) and its neighbours."""
-    sizes = [(34, 18), (5, 3), (7, 5), (1, 1), (9, 2), (16, 7), (23, 9)]
+    an even width), a span boundary (16 pixels) and its neighbours."""
+    sizes = [(34, 18), (5, 3), (7, 5), (1, 1), (9, 2), (16, 7), (23, 9), (32, 2), (33, 3)]
     return [(w, h) for w, h in sizes if not (dst in PACKED_422 and w % 2)]
 
 
-def rgb_picture(fmt, width, height, seed, device=False, pad=0):
+def rgb_picture(fmt, width, height, seed, device=False, pad=0, align=1):
     """A source Picture of seeded bytes (alpha included)."""
-    pic = Picture(fmt, width, height, pad=pad)
+    pic = Picture(fmt, width, height, pad=pad, align=align)
     rng = np.random.default_rng(seed)
     pic.buf[:] = rng.integers(0, 256, pic.nbytes, dtype=np.uint8)
     if device:
-        dev = Picture(fmt, width, height, device=True, pad=pad)
+        dev = Picture(fmt, width, height, device=True, pad=pad, align=align)
         import torch
         dev.buf.copy_(torch.from_numpy(pic.buf))
         return pic, dev

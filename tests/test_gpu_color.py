@@ -22,17 +22,17 @@ pytestmark = pytest.mark.gpu
 FILL = 0xA5
 
 
-def on_gpu(torch, src_host, dst_fmt, matrix, range_, pad=0, offset=0):
+def on_gpu(torch, src_host, dst_fmt, matrix, range_, pad=0, offset=0, align=1):
     """Converts a host source Picture on the GPU; returns the destination Picture (host copy
     in .buf) and its device buffer. `offset` shifts both device buffers off their 256-byte
-    alignment."""
+    alignment; `align` rounds the destination's pitches up to its multiple."""
     w, h = src_host.width, src_host.height
     src_dev = Picture(src_host.format, w, h, device=True, pad=(src_host.layout[0][1] - src_host.layout[0][2]))
     raw = torch.full((src_dev.nbytes + offset,), 0, dtype=torch.uint8, device="cuda")
     raw[offset:] = torch.from_numpy(src_host.buf)
     sview = src_dev.view()
     sview.planes[0].data = raw.data_ptr() + offset
-    dst = Picture(dst_fmt, w, h, pad=pad, fill=FILL)
+    dst = Picture(dst_fmt, w, h, pad=pad, fill=FILL, align=align)
     draw = torch.full((dst.nbytes + offset,), FILL, dtype=torch.uint8, device="cuda")
     dview = dst.view()
     base = dst.buf.ctypes.data
@@ -44,8 +44,8 @@ def on_gpu(torch, src_host, dst_fmt, matrix, range_, pad=0, offset=0):
     return dst
 
 
-def oracle_of(src_host, dst_fmt, matrix, range_, pad=0):
-    want = Picture(dst_fmt, src_host.width, src_host.height, pad=pad, fill=FILL)
+def oracle_of(src_host, dst_fmt, matrix, range_, pad=0, align=1):
+    want = Picture(dst_fmt, src_host.width, src_host.height, pad=pad, fill=FILL, align=align)
     oracle_lib.color_convert(src_host, want, matrix, range_)
     return want
 
@@ -69,6 +69,20 @@ def test_padded_and_unaligned_planes(torch_gpu, pad, offset):
         got = on_gpu(torch_gpu, source, dst, _abi.MATRIX_BT709, _abi.RANGE_LIMITED, pad=pad, offset=offset)
         want = oracle_of(source, dst, _abi.MATRIX_BT709, _abi.RANGE_LIMITED, pad=pad)
         assert np.array_equal(got.buf, want.buf), (src, dst, pad, offset)
+
+
+@pytest.mark.parametrize("dst", cc.DESTINATIONS)
+def test_vector_spans_beside_ragged_edges(torch_gpu, dst):
+    """Pitches rounded up to 16 bytes put the interior spans on the vector loads and stores;
+    the last span of a row (a width that is not a multiple of 16) and, for 4:2:0, the odd
+    last row take the byte-wise path. Every byte must agree across the seam."""
+    for src, (w, h) in itertools.product(cc.SOURCES, cc.VECTOR_SIZES):
+        if dst in cc.PACKED_422 and w % 2:
+            continue
+        source = cc.rgb_picture(src, w, h, seed=11 * w + h + src, align=16)
+        got = on_gpu(torch_gpu, source, dst, _abi.MATRIX_BT601, _abi.RANGE_FULL, align=16)
+        want = oracle_of(source, dst, _abi.MATRIX_BT601, _abi.RANGE_FULL, align=16)
+        assert np.array_equal(got.buf, want.buf), (src, dst, w, h)
 
 
 @pytest.mark.skipif(not oracle_lib.cc_ref_available(), reason="reference converter not built here")
