@@ -112,6 +112,19 @@ __device__ __forceinline__ int color_of(int channel) {
 }
 __device__ __forceinline__ float clamp01(float v) { return v < 0.0F ? 0.0F : v > 1.0F ? 1.0F : v; }
 
+// p * a + b for p a power of two, in one instruction. The product by a power of
+// two is exact (the values here neither overflow nor reach the subnormals), so
+// the single rounding of the fused operation is the rounding of the reference's
+// sum: bit for bit the unfused p * a + b.
+__device__ __forceinline__ float exact_fma(float p, float a, float b) { return __builtin_fmaf(p, a, b); }
+
+// to_fixed (isp_math.hpp:245-247). (Folding 2^20 into an exact_fma holds the
+// constant in a scalar register, which in the main pass costs more in scalar
+// spills than the product saves.)
+__device__ __forceinline__ unsigned int to_fixed(float v) {
+  return static_cast<unsigned int>(v * 1048576.0F + 0.5F);
+}
+
 // load_raw (isp_math.hpp:101-119), byte-wise: the edges and unaligned frames.
 __device__ __forceinline__ int load_raw(const std::uint8_t* row, int format, int x) {
   switch (format) {
@@ -464,7 +477,7 @@ __device__ __forceinline__ void awb_chunk_fast(const FrameDesc& f, const AwbCtx&
       v = v > 0.0F ? v : 0.0F;
       if constexpr (kLsc) v = v * gain[r][col];
       v = v < 4.0F ? v : 4.0F;
-      fx[k] = static_cast<unsigned int>(v * 1048576.0F + 0.5F);
+      fx[k] = to_fixed(v);
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) pos[k] += ok ? fx[k] : 0U;
@@ -553,7 +566,7 @@ __device__ __forceinline__ void awb_tile(const FrameDesc& f, const AwbCtx& c, Aw
         v = v * gain;
       }
       v = v < 4.0F ? v : 4.0F;
-      pos[k] += static_cast<unsigned int>(v * 1048576.0F + 0.5F);
+      pos[k] += to_fixed(v);
       if (c.patch) {
         const int b = static_cast<int>(v * static_cast<float>(kAwbBins));
         atomicAdd(&fl.hist[c.colour[k] * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
@@ -725,15 +738,18 @@ __device__ __forceinline__ void demosaic_pair(const float (&w)[kRows][kCols], in
     const float n24 = w[cy][cx + 2];
     const float diagonal1 = w[cy - 1][cx - 1] + w[cy - 1][cx + 1] + w[cy + 1][cx - 1] + w[cy + 1][cx + 1];
     const float vertical2 = n02 + n42;
+    // The reference's sums in its order, each product by a power of two folded
+    // into its sum (exact_fma); 4c + 2 axial1 is 2 (2c + axial1), its rounding
+    // the double of that sum's.
     if (rb) {
       const float axial1 = n12 + n32 + n21 + n23;
       const float axial2 = n02 + n42 + n20 + n24;
-      *first = clamp01((4.0F * c + 2.0F * axial1 - axial2) * 0.125F);
-      *second = clamp01((6.0F * c + 2.0F * diagonal1 - 1.5F * axial2) * 0.125F);
+      *first = clamp01(exact_fma(2.0F, exact_fma(2.0F, c, axial1), -axial2) * 0.125F);
+      *second = clamp01((exact_fma(2.0F, diagonal1, 6.0F * c) - 1.5F * axial2) * 0.125F);
     } else {
       const float horizontal2 = n20 + n24;
-      *first = clamp01((5.0F * c + 4.0F * (n21 + n23) - horizontal2 - diagonal1 + 0.5F * vertical2) * 0.125F);
-      *second = clamp01((5.0F * c + 4.0F * (n12 + n32) - vertical2 - diagonal1 + 0.5F * horizontal2) * 0.125F);
+      *first = clamp01(exact_fma(0.5F, vertical2, exact_fma(4.0F, n21 + n23, 5.0F * c) - horizontal2 - diagonal1) * 0.125F);
+      *second = clamp01(exact_fma(0.5F, horizontal2, exact_fma(4.0F, n12 + n32, 5.0F * c) - vertical2 - diagonal1) * 0.125F);
     }
   }
 }
@@ -1162,9 +1178,11 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const float cc = m[j + 1][i + 2];
-          const float blur = (4.0F * cc + 2.0F * (m[j][i + 2] + m[j + 2][i + 2] + m[j + 1][i + 1] + m[j + 1][i + 3]) +
-                              (m[j][i + 1] + m[j][i + 3] + m[j + 2][i + 1] + m[j + 2][i + 3])) *
-                             0.0625F;
+          // (4 cc + 2 axial + diagonal) / 16 with the powers of two folded:
+          // 4 cc + 2 axial is 2 (2 cc + axial), exactly.
+          const float axial = m[j][i + 2] + m[j + 2][i + 2] + m[j + 1][i + 1] + m[j + 1][i + 3];
+          const float diagonal = m[j][i + 1] + m[j][i + 3] + m[j + 2][i + 1] + m[j + 2][i + 3];
+          const float blur = exact_fma(2.0F, exact_fma(2.0F, cc, axial), diagonal) * 0.0625F;
           rgb[j][i][k] = clamp01(cc + f.sharpen * (cc - blur));
         }
       }
@@ -1214,7 +1232,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
       }
       if (c.want_stats && (full_x || xo + i < W)) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) sums[k] += static_cast<unsigned int>(lin[i][k] * 1048576.0F + 0.5F);
+        for (int k = 0; k < 3; ++k) sums[k] += to_fixed(lin[i][k]);
       }
     }
     if (c.lut_lds) {

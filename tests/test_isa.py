@@ -1,8 +1,11 @@
 """The kernels' gfx950 assembly keeps the reference's float semantics: no
 contracted (fused) single-precision operation outside the IEEE division
-expansion (v_div_scale .. v_div_fixup), and no packed fp32 arithmetic."""
+expansion (v_div_scale .. v_div_fixup) but those whose product is exact (one
+factor a power of two, which the kernels fold into a sum on purpose), and no
+packed fp32 arithmetic."""
 import os
 import re
+import struct
 import subprocess
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -21,7 +24,7 @@ def asm_text(name="isp_kernels.s"):
     return _BUILT[name]
 
 
-FUSED = re.compile(r"\bv_(fma|fmac|mac|mad|fma_mix|fma_mixlo|fma_mixhi|pk_fma)_f32\b|\bv_fma_mix")
+FUSED = re.compile(r"\bv_(fma|fmac|mac|mad|fmamk|fmaak|madmk|madak|fma_mix|fma_mixlo|fma_mixhi|pk_fma)_f32\b|\bv_fma_mix")
 PACKED = re.compile(r"\bv_pk_(add|mul|fma)_f32\b")
 
 
@@ -36,13 +39,73 @@ def in_division(lines, i):
     return "v_rcp_iflag_f32" in "\n".join(lines[max(0, i - 8):i])
 
 
+INLINE_POW2 = {"0.5", "-0.5", "1.0", "-1.0", "2.0", "-2.0", "4.0", "-4.0"}
+
+
+def pow2_bits(text):
+    """Whether a 32-bit constant's float value is plus or minus a power of two."""
+    try:
+        bits = int(text, 0) & 0xFFFFFFFF
+    except ValueError:
+        return False
+    value = struct.unpack("<f", struct.pack("<I", bits))[0]
+    exponent = (bits >> 23) & 0xFF
+    return value != 0.0 and (bits & 0x7FFFFF) == 0 and 0 < exponent < 255
+
+
+def exact_product(lines, i):
+    """A fused op one of whose factors is a power of two: an inline constant, a literal, or an
+    SGPR whose latest s_mov_b32 above (in the same function) set one. Its product is exact,
+    so it rounds as the unfused sum does."""
+    mnemonic, _, rest = lines[i].strip().partition(" ")
+    ops = [o.strip() for o in rest.split(";")[0].split(",")]
+    base = re.sub(r"_e(32|64)$", "", mnemonic)
+    if base in ("v_fmamk_f32", "v_madmk_f32"):
+        factors = [ops[1], ops[2]]
+    elif len(ops) >= 3:
+        factors = [ops[1], ops[2]]
+    else:
+        return False
+    for f in factors:
+        f = f.lstrip("-").strip("|")
+        if f in INLINE_POW2 or (f.startswith("0x") and pow2_bits(f)):
+            return True
+        if re.fullmatch(r"s\d+", f):
+            for j in range(i - 1, -1, -1):
+                line = lines[j].strip()
+                if line.startswith(".Lfunc_end") or line.endswith(":") and not line.startswith("."):
+                    if not line.startswith(".LBB"):
+                        break
+                m = re.match(r"s_mov_b32 (s\d+), (\S+)", line)
+                if m and m.group(1) == f:
+                    if m.group(2) in INLINE_POW2 or pow2_bits(m.group(2)):
+                        return True
+                    break
+                if re.match(rf"s_\w+ {f},", line):
+                    break
+    return False
+
+
 def test_no_contraction_in_kernels():
+    exact = 0
     for name in FILES:
         lines = asm_text(name).splitlines()
         for i, line in enumerate(lines):
             assert not PACKED.search(line), f"{name}:{i + 1}: {line.strip()}"
-            if FUSED.search(line):
-                assert in_division(lines, i), f"{name}:{i + 1}: fused op outside a division: {line.strip()}"
+            if FUSED.search(line) and not in_division(lines, i):
+                assert exact_product(lines, i), f"{name}:{i + 1}: fused op outside a division: {line.strip()}"
+                exact += 1
+    assert exact > 0  # the demosaic and sharpening fold their powers of two
+
+
+def test_the_exact_product_rule():
+    lines = ["s_mov_b32 s4, 0x49800000", "v_add_f32_e32 v1, v2, v3", "v_fma_f32 v0, v1, s4, 0.5",
+             "v_fmac_f32_e32 v0, 2.0, v1", "v_fmac_f32_e32 v0, v3, v1", "v_fma_f32 v0, v1, 0x40400000, v2",
+             "v_fmamk_f32 v0, v1, 0x4f800000, v2", "s_mov_b32 s4, 0x3f000001", "v_fma_f32 v0, v1, s4, v2"]
+    assert exact_product(lines, 2) and exact_product(lines, 3) and exact_product(lines, 6)
+    assert not exact_product(lines, 4)  # two registers
+    assert not exact_product(lines, 5)  # 3.0
+    assert not exact_product(lines, 8)  # s4 was reset to a non-power of two
 
 
 def test_kernels_make_no_calls_and_use_no_scratch():
