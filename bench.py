@@ -114,6 +114,18 @@ def percentile(values, q):
     return s[k]
 
 
+def cpu_model():
+    """The host CPU's model name, as /proc/cpuinfo gives it."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(frame, params, out_format, frames):
     """The reference CPU backend (compiled from its own sources) when present, else the
     oracle; 1 thread, one warm-up call then the mean of `frames` calls
@@ -134,7 +146,7 @@ def cpu_baseline(frame, params, out_format, frames):
     dt = (time.perf_counter() - t0) / frames
     out["single"] = dict(value=pixels / dt / 1e9, unit="Gpix/s", cores=1, kind=kind,
                          sample=f"{frames} frames of the 3840x2160 RAW12 workload after 1 warm-up, "
-                                f"{dt * 1e3:.1f} ms/frame")
+                                f"{dt * 1e3:.1f} ms/frame", cpu=cpu_model(), nproc=os.cpu_count())
     threads = min(16, os.cpu_count() or 1)
     orc = oracle_lib.OraclePipeline(params)
     orc.process(frame, out_format, True, (10_000_000, 2.0), threads=threads)
@@ -364,11 +376,13 @@ def served_report(seconds):
     return out
 
 
-def source_and_check(launches):
+def source_and_check(launches, cpu=True):
     """The device raw source (isp_synthetic_render_device) and the GPU frame checker
     (isp_check_frame_device) on the reference demo's chart camera: each kernel's average
     launch time from HIP events over `launches` launches on one stream, at 4K RAW12 and 1080p
-    u16, with the checked frame's PSNR and mismatched bins."""
+    u16, with the checked frame's PSNR and mismatched bins. With `cpu`, the CPU paths beside
+    them, one thread, one call each: the reference's host renderer compiled from its sources
+    (synthetic_raw.cpp) and the oracle's checker (a restatement of frame_check.cu)."""
     import torch
     import holoscan_isp as isp
     from holoscan_isp import _abi
@@ -406,14 +420,45 @@ def source_and_check(launches):
         out[name] = {"render_us": round(render_us, 2), "render_gpix_s": round(w * h / render_us / 1e3, 2),
                      "check_us": round(check_us, 2), "check_gpix_s": round(w * h / check_us / 1e3, 2),
                      "psnr_db": round(isp.frame_psnr(r, w, h), 3), "histogram_mismatches": r.histogram_mismatches}
+        if cpu:
+            out[name]["cpu"] = cpu_source_and_check(cam, params, rgb, hist, pitch)
         pipe.close()
     return out
 
 
-def color_conversion(launches):
+def cpu_source_and_check(cam, params, rgb, hist, pitch):
+    """One call each of the host renderer (the reference's when oracle/_ref is built, else
+    this library's host render) and the oracle's frame check, on the frame just checked."""
+    import ctypes as C
+    import numpy as np
+    import holoscan_isp as isp
+    import oracle_lib
+    w, h = cam.width, cam.height
+    buf = np.zeros(pitch * h, dtype=np.uint8)
+    t0 = time.perf_counter()
+    if oracle_lib.ref_available():
+        kind = "reference"
+        if oracle_lib.ref().ref_synthetic_render(C.byref(cam), buf.ctypes.data, pitch) != 0:
+            raise RuntimeError("ref_synthetic_render failed")
+    else:
+        kind = "port"
+        isp.render_synthetic(cam)
+    render_s = time.perf_counter() - t0
+    truth, lut = isp.synthetic_truth(cam), isp.tone_lut(params)
+    host_rgb, host_hist = rgb.cpu().numpy().reshape(h, w, 3), hist.cpu().numpy().view(np.uint32)
+    t0 = time.perf_counter()
+    oracle_lib.frame_check(truth, lut, host_rgb, host_hist)
+    check_s = time.perf_counter() - t0
+    return {"render_ms": round(render_s * 1e3, 1), "render_kind": kind, "check_ms": round(check_s * 1e3, 1),
+            "check_kind": "port", "cores": 1}
+
+
+def color_conversion(launches, cpu=True):
     """The conversion after the ISP (isp_color_convert_device) on a 4K frame: each kernel's
     average launch time from HIP events over `launches` launches, and its rate against the
-    bytes it must move (read the RGB frame once, write the YCbCr planes once)."""
+    bytes it must move (read the RGB frame once, write the YCbCr planes once). With `cpu`, the
+    colour module's own CPU converter (compiled from its sources) beside it, one thread, one
+    call each."""
     import torch
     import numpy as np
     from holoscan_isp import _abi, color
@@ -444,6 +489,17 @@ def color_conversion(launches):
         moved = src.nbytes + dst.nbytes
         out[name] = {"us": round(us, 2), "bytes": moved, "gb_s": round(moved / us / 1e3, 1),
                      "hbm_frac": round(moved / us / 1e3 / HBM_PEAK_GBS, 4)}
+        if cpu:
+            import oracle_lib
+            if oracle_lib.cc_ref_available():
+                host_src = color.Picture(src_fmt, w, h)
+                host_src.buf[:] = src.buf.cpu().numpy()
+                host_dst = color.Picture(dst_fmt, w, h)
+                t0 = time.perf_counter()
+                oracle_lib.ref_color_convert(host_src, host_dst, _abi.MATRIX_BT709, _abi.RANGE_LIMITED)
+                out[name]["cpu_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+                out[name]["cpu_kind"] = "reference"
+                out[name]["bit_exact"] = bool(np.array_equal(host_dst.buf, dst.buf.cpu().numpy()))
     return out
 
 
@@ -562,9 +618,9 @@ def main():
     if args.served_seconds > 0:
         result["served_32_cameras"] = served_report(args.served_seconds)
     if args.source_launches > 0:
-        result["device_source_and_check"] = source_and_check(args.source_launches)
+        result["device_source_and_check"] = source_and_check(args.source_launches, cpu=not args.no_cpu)
     if args.color_launches > 0:
-        result["color_conversion_4k"] = color_conversion(args.color_launches)
+        result["color_conversion_4k"] = color_conversion(args.color_launches, cpu=not args.no_cpu)
     if not args.no_cpu:
         base = cpu_baseline(frame, params, out_format, args.cpu_frames)
         result["cpu_baseline"] = base["single"]
