@@ -41,6 +41,8 @@ def parse():
                     help="timed launches of the device raw source and the frame checker (0: skip)")
     ap.add_argument("--color-launches", type=int, default=100,
                     help="timed launches of each 4K colour conversion after the ISP (0: skip)")
+    ap.add_argument("--config1-frames", type=int, default=500,
+                    help="timed frames of the 1080p u16 workload (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times): the reference "
                          "point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -453,6 +455,75 @@ def cpu_source_and_check(cam, params, rgb, hist, pitch):
             "check_kind": "port", "cores": 1}
 
 
+def config1_frames(frames, cpu_frames):
+    """The 1080p u16 workload (1920x1080 RGGB, 12 bits in 16, bilinear, manual white balance,
+    lens shading, RGB8 + histograms + statistics): per-frame launches back to back, Gpix/s and
+    HBM GB/s at its compulsory 5 B/px, per-frame p50/p99 from HIP events, a bit-exact check
+    of a fresh pipeline's frame against the oracle, and the reference CPU backend (oracle/_ref)
+    on one thread beside it."""
+    import numpy as np
+    import torch
+    import holoscan_isp as isp
+    import cases
+    import gpu_support as gs
+    import oracle_lib
+    scene, params, out_format = cases.config1()
+    frame = isp.render_scene(scene)
+    w, h = frame.width, frame.height
+    d_raw = torch.from_numpy(frame.data).to("cuda")
+    out = torch.empty(3 * w * h, dtype=torch.uint8, device="cuda")
+    hist = torch.empty(4 * 256, dtype=torch.int32, device="cuda")
+    stats = torch.empty(12, dtype=torch.float32, device="cuda")
+    pipe = isp.IspPipeline(params)
+    view = frame.view(d_raw)
+    stream = torch.cuda.current_stream()
+    exposure = (10_000_000, 2.0)
+
+    def step():
+        pipe.process(view, out, 3 * w, out_format, hist, stats, exposure, stream.cuda_stream)
+
+    for _ in range(20):
+        step()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(frames):
+        step()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / frames
+    per = []
+    for _ in range(frames):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        step()
+        b.record(stream)
+        per.append((a, b))
+    torch.cuda.synchronize()
+    per = [a.elapsed_time(b) for a, b in per]
+    pipe.close()
+    g = gs.run(isp.IspPipeline(params), frame, out_format, True, exposure, d_raw=d_raw)
+    o = oracle_lib.OraclePipeline(params).process(frame, out_format, True, exposure,
+                                                 threads=min(16, os.cpu_count() or 1))
+    exact = bool(gs.mismatches(g, o) == 0 and np.array_equal(g.histogram, o.histogram) and gs.same_stats(g, o))
+    result = {"workload": "1920x1080 RGGB u16 12-bit, bilinear, manual white balance, lens shading, CCM, "
+                          "gamma 2.2, RGB8 + histograms + stats", "frames": frames,
+              "ms_per_frame": round(ms, 4), "gpix_s": round(w * h / ms / 1e6, 3),
+              "hbm_gbs": round(w * h * 5 / ms / 1e6, 1), "frame_ms_p50": round(percentile(per, 50), 4),
+              "frame_ms_p99": round(percentile(per, 99), 4), "bit_exact": exact}
+    if cpu_frames > 0 and oracle_lib.ref_available():
+        ref = oracle_lib.RefPipeline(params)
+        ref.process(frame, out_format, True, exposure)
+        t0 = time.perf_counter()
+        for _ in range(cpu_frames):
+            ref.process(frame, out_format, True, exposure)
+        dt = (time.perf_counter() - t0) / cpu_frames
+        result["cpu_baseline"] = {"value": round(w * h / dt / 1e9, 5), "unit": "Gpix/s", "cores": 1,
+                                  "kind": "reference", "ms_per_frame": round(dt * 1e3, 1),
+                                  "sample": f"{cpu_frames} frames after 1 warm-up"}
+    return result
+
+
 def color_conversion(launches, cpu=True):
     """The conversion after the ISP (isp_color_convert_device) on a 4K frame: each kernel's
     average launch time from HIP events over `launches` launches, and its rate against the
@@ -619,6 +690,8 @@ def main():
         result["served_32_cameras"] = served_report(args.served_seconds)
     if args.source_launches > 0:
         result["device_source_and_check"] = source_and_check(args.source_launches, cpu=not args.no_cpu)
+    if args.config1_frames > 0:
+        result["config1_1080p_u16"] = config1_frames(args.config1_frames, 0 if args.no_cpu else 10)
     if args.color_launches > 0:
         result["color_conversion_4k"] = color_conversion(args.color_launches, cpu=not args.no_cpu)
     if not args.no_cpu:
