@@ -112,10 +112,12 @@ __device__ __forceinline__ int color_of(int channel) {
 }
 __device__ __forceinline__ float clamp01(float v) { return v < 0.0F ? 0.0F : v > 1.0F ? 1.0F : v; }
 
-// p * a + b for p a power of two, in one instruction. The product by a power of
-// two is exact (the values here neither overflow nor reach the subnormals), so
-// the single rounding of the fused operation is the rounding of the reference's
-// sum: bit for bit the unfused p * a + b.
+// p * a + b for p = 2 or 4, in one instruction. Scaling up by a power of two
+// is exact for every float short of overflow (the values here are at most a few
+// units), subnormals included, so the single rounding of the fused operation is
+// the rounding of the reference's sum: bit for bit the unfused p * a + b.
+// (Halving is not exact for a subnormal with its last bit set, so products by
+// 0.5 stay unfused.)
 __device__ __forceinline__ float exact_fma(float p, float a, float b) { return __builtin_fmaf(p, a, b); }
 
 // to_fixed (isp_math.hpp:245-247). (Folding 2^20 into an exact_fma holds the
@@ -738,9 +740,9 @@ __device__ __forceinline__ void demosaic_pair(const float (&w)[kRows][kCols], in
     const float n24 = w[cy][cx + 2];
     const float diagonal1 = w[cy - 1][cx - 1] + w[cy - 1][cx + 1] + w[cy + 1][cx - 1] + w[cy + 1][cx + 1];
     const float vertical2 = n02 + n42;
-    // The reference's sums in its order, each product by a power of two folded
-    // into its sum (exact_fma); 4c + 2 axial1 is 2 (2c + axial1), its rounding
-    // the double of that sum's.
+    // The reference's sums in its order, each product by 2 or 4 folded into its
+    // sum (exact_fma); 4c + 2 axial1 is 2 (2c + axial1), its rounding the double
+    // of that sum's (both exact when the sum is subnormal).
     if (rb) {
       const float axial1 = n12 + n32 + n21 + n23;
       const float axial2 = n02 + n42 + n20 + n24;
@@ -748,8 +750,8 @@ __device__ __forceinline__ void demosaic_pair(const float (&w)[kRows][kCols], in
       *second = clamp01((exact_fma(2.0F, diagonal1, 6.0F * c) - 1.5F * axial2) * 0.125F);
     } else {
       const float horizontal2 = n20 + n24;
-      *first = clamp01(exact_fma(0.5F, vertical2, exact_fma(4.0F, n21 + n23, 5.0F * c) - horizontal2 - diagonal1) * 0.125F);
-      *second = clamp01(exact_fma(0.5F, horizontal2, exact_fma(4.0F, n12 + n32, 5.0F * c) - vertical2 - diagonal1) * 0.125F);
+      *first = clamp01((exact_fma(4.0F, n21 + n23, 5.0F * c) - horizontal2 - diagonal1 + 0.5F * vertical2) * 0.125F);
+      *second = clamp01((exact_fma(4.0F, n12 + n32, 5.0F * c) - vertical2 - diagonal1 + 0.5F * horizontal2) * 0.125F);
     }
   }
 }
@@ -780,13 +782,16 @@ __device__ __forceinline__ void assign_rgb(int ch, float c, float first, float s
   }
 }
 
-// tone (isp_math.hpp:235-243). f - (float)(int)f equals fract(f) exactly for
-// f >= 0; the index is kept in the table for the top entry, which the select
-// then replaces by lut[n1] as the reference returns it.
+// tone (isp_math.hpp:235-243). The fraction is the reference's f - (float)i:
+// v_fract_f32 gives the same for every f >= 0 but not for f = -0, which the
+// colour matrix's clamp passes on (a row of negative weights on a black pixel)
+// and which keeps its sign through a table whose first entry is -0. The index is
+// kept in the table for the top entry, which the select then replaces by lut[n1]
+// as the reference returns it.
 __device__ __forceinline__ float tone(const float* lut, int n1, float top, float x) {
   const float f = x * static_cast<float>(n1);
   const int i = static_cast<int>(f);
-  const float t = __builtin_amdgcn_fractf(f);
+  const float t = f - static_cast<float>(i);
   const unsigned int ii = static_cast<unsigned int>(i < n1 - 1 ? i : n1 - 1);
   const float a = lut[ii];
   const float b = lut[ii + 1U];

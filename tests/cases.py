@@ -85,6 +85,42 @@ def matches_cpu_cases():
     return cases
 
 
+def float_edge_cases():
+    """Frames whose float path reaches the edges of IEEE arithmetic, with settings validation
+    accepts: (name, frame, params, out_format).
+    - subnormal samples: white balance gains of about 1e-39 (validation asks only > 0) make
+      every linear sample subnormal, through demosaic, sharpening and the colour matrix;
+    - signed zeros: a random mosaic with many samples below black, a colour matrix whose
+      last row is all negative (so black pixels give -0 there, which clamp01 keeps), and a
+      user tone table whose first entry is -0.0 (validation accepts it as 0), so a -0 can
+      reach the RGB32F output."""
+    out = []
+    cfg = camera(_abi.GRBG, FORMATS[2], 67, 37)
+    frame = isp.render_synthetic(cfg)
+    p = isp.synthetic_params(cfg, 5)
+    p.wb_gains = (3e-39, 1e-39, 2e-39)
+    p.demosaic, p.sharpen = _abi.MALVAR, 0.7
+    p.ccm = (0.9, 0.2, -0.1, 0.05, 0.8, 0.15, -0.2, 0.1, 1.1)
+    p.tone_curve = _abi.TONE_LINEAR
+    out.append(("subnormal_malvar", frame, p, _abi.OUT_RGB32F))
+    q = isp.synthetic_params(cfg, 5)
+    q.wb_gains = (5e-39, 2e-39, 1e-39)
+    q.demosaic, q.sharpen = _abi.BILINEAR, 2.0
+    q.tone_curve = _abi.TONE_GAMMA
+    out.append(("subnormal_bilinear", frame, q, _abi.OUT_RGB16))
+    rng = np.random.default_rng(77)
+    img = rng.integers(0, 600, size=(29, 45), dtype=np.uint16)
+    mosaic = isp.RawFrame(img.view(np.uint8).ravel(), 45, 29, 90, _abi.RAW_U16, 12, _abi.RGGB)
+    lut = np.linspace(0.0, 1.0, 257, dtype=np.float32)
+    lut[0] = np.float32(-0.0)
+    for method, sharpen in ((_abi.BILINEAR, 0.0), (_abi.MALVAR, 0.0), (_abi.MALVAR, 1.5)):
+        r = isp.IspParams(black_level=400.0, wb_gains=(1.0, 1.0, 1.0), demosaic=method, sharpen=sharpen,
+                          ccm=(1.0, 0.0, 0.0, 0.0, 1.0, 0.0, -0.25, -0.5, -0.25),
+                          tone_curve=_abi.TONE_LUT, lut=[float(v) for v in lut])
+        out.append((f"signed_zero_{method}_{sharpen}", mosaic, r, _abi.OUT_RGB32F))
+    return out
+
+
 def baseline_scene(width, height, fmt, bits, pattern, black, seed, **kw):
     return isp.scene_config(width=width, height=height, format=fmt, bit_depth=bits, pattern=pattern,
                             black_level=black, seed=seed, **kw)

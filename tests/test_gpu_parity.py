@@ -86,6 +86,16 @@ def test_tiny_sizes(torch_gpu, size, pattern, method):
     compare(f"{w}x{h}", g, c)
 
 
+@pytest.mark.parametrize("case", cases.float_edge_cases(), ids=lambda c: c[0])
+def test_float_edges(torch_gpu, case):
+    """Subnormal samples (tiny white balance gains) and signed zeros reaching an RGB32F output,
+    bit for bit."""
+    name, frame, params, out_format = case
+    gpu, cpu = isp.IspPipeline(params), OraclePipeline(params)
+    for _ in range(2):
+        compare(name, gs.run(gpu, frame, out_format, True, EXPOSURE), cpu.process(frame, out_format, True, EXPOSURE))
+
+
 def test_random_mosaics(torch_gpu):
     """LCG mosaics (cpu_reference_test.cpp:57-66) at odd sizes, every option on."""
     rng = np.random.default_rng(12345)

@@ -76,6 +76,30 @@ def test_oracle_matches_reference_matrix(fi, pattern, method, sharpen, wb):
 
 
 @needs_ref
+@pytest.mark.parametrize("case", cases.float_edge_cases(), ids=lambda c: c[0])
+def test_oracle_matches_reference_at_float_edges(case):
+    """Subnormal samples and signed zeros: the oracle's bits are the compiled reference's."""
+    name, frame, params, out_format = case
+    a = OraclePipeline(params).process(frame, out_format, True, (10_000_000, 2.0))
+    b = oracle_lib.RefPipeline(params).process(frame, out_format, True, (10_000_000, 2.0))
+    assert np.array_equal(a.image, b.image), name
+    assert np.array_equal(a.histogram, b.histogram), name
+    assert np.array_equal(a.stats.view(np.uint32), b.stats.view(np.uint32)), name
+
+
+def test_float_edge_cases_reach_the_edges():
+    """The cases do what they are for: subnormal linear values, and -0.0 in an RGB32F output."""
+    edge = {name: (frame, params, fmt) for name, frame, params, fmt in cases.float_edge_cases()}
+    frame, params, fmt = edge["subnormal_malvar"]
+    img = OraclePipeline(params).process(frame, fmt, True, (10_000_000, 2.0)).image.view(np.float32)
+    nonzero = np.abs(img[img != 0])
+    assert nonzero.size > 0 and nonzero.max() < np.float32(1.2e-38)
+    frame, params, fmt = edge["signed_zero_1_0.0"]
+    img = OraclePipeline(params).process(frame, fmt, True, (10_000_000, 2.0)).image.view(np.float32)
+    assert np.any((img == 0) & np.signbit(img))
+
+
+@needs_ref
 def test_oracle_threads_match_reference():
     """Row-band threads change nothing: the sums are integers."""
     scene, params, out_format = cases.config2(384, 216, seed=3)
