@@ -110,7 +110,10 @@ __device__ __forceinline__ int cfa_channel(int cfa, int p) { return (cfa >> (2 *
 __device__ __forceinline__ int color_of(int channel) {
   return channel == ISP_CHANNEL_R ? 0 : channel == ISP_CHANNEL_B ? 2 : 1;
 }
-__device__ __forceinline__ float clamp01(float v) { return v < 0.0F ? 0.0F : v > 1.0F ? 1.0F : v; }
+// clamp01 (isp_math.hpp:157), bit for bit for every finite v, -0 included (kept,
+// as the reference keeps it): the upper cut as a v_min_f32, the lower as a
+// select, one operation fewer than two selects.
+__device__ __forceinline__ float clamp01(float v) { return v < 0.0F ? 0.0F : fminf(v, 1.0F); }
 
 // p * a + b for p = 2 or 4, in one instruction. Scaling up by a power of two
 // is exact for every float short of overflow (the values here are at most a few
