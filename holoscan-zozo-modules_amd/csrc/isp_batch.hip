@@ -261,7 +261,7 @@ struct isp_batch {
       std::ostringstream o;
       o << fr.raw.width << 'x' << fr.raw.height << ':' << int(fr.raw.format) << ':' << int(c.demosaic) << ':'
         << (c.sharpen > 0.0F ? 1 : 0) << ':' << int(c.white_balance) << ':' << int(fr.out.format) << ':'
-        << fr.pipeline->params.bins() << ':' << (fr.pipeline->lut.size() > 4100 ? 1 : 0);
+        << fr.pipeline->params.bins() << ':' << (fr.pipeline->lut.size() + 1 > 4100 ? 1 : 0);
       parts.push_back(o.str());
     }
     std::sort(parts.begin(), parts.end());

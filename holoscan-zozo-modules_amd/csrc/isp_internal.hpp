@@ -72,7 +72,7 @@ struct FrameDesc {
   float lsc_sx;
   float lsc_sy;
   float ccm[9];
-  const float* lut;  // device table
+  const float* lut;  // device table: lut_size entries, then lut[lut_size - 1] - 1
   int lut_size;
   float sharpen;
   std::uint8_t* out;

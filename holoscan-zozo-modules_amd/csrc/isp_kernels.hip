@@ -788,21 +788,29 @@ __device__ __forceinline__ void assign_rgb(int ch, float c, float first, float s
 // tone (isp_math.hpp:235-243). The fraction is the reference's f - (float)i:
 // v_fract_f32 gives the same for every f >= 0 but not for f = -0, which the
 // colour matrix's clamp passes on (a row of negative weights on a black pixel)
-// and which keeps its sign through a table whose first entry is -0. The index is
-// kept in the table for the top entry, which the select then replaces by lut[n1]
-// as the reference returns it.
-__device__ __forceinline__ float tone(const float* lut, int n1, float top, float x) {
+// and which keeps its sign through a table whose first entry is -0. The table
+// carries one entry more, lut[n1 + 1] = lut[n1] - 1: at x = 1 (the only x with
+// i = n1, since x <= 1), t = +0 and b - a < 0, so a + t * (b - a) is lut[n1]
+// plus -0, which is lut[n1] exactly, -0 included, as the reference returns it;
+// below, the entries read are the reference's. No clamp of the index, no
+// select. T is the table's type: an LDS table is read through an LDS pointer,
+// so its two entries are a ds_read2_b32 (through a generic pointer the
+// compiler merges the LDS and HBM paths into flat loads, and an 8-byte flat
+// load of an odd entry of LDS is slow).
+template <typename T>
+__device__ __forceinline__ float tone(T* lut, int n1, float x) {
   const float f = x * static_cast<float>(n1);
   const int i = static_cast<int>(f);
   const float t = f - static_cast<float>(i);
-  const unsigned int ii = static_cast<unsigned int>(i < n1 - 1 ? i : n1 - 1);
-  const float a = lut[ii];
-  const float b = lut[ii + 1U];
-  const float v = a + t * (b - a);
-  return i >= n1 ? top : v;
+  const float a = lut[i];
+  const float b = lut[i + 1];
+  return a + t * (b - a);
 }
 
-constexpr int kLdsLut = 4100;  // tone tables up to this size are read from LDS
+// A float in LDS, addressed as such.
+typedef __attribute__((address_space(3))) const float LdsFloat;
+
+constexpr int kLdsLut = 4100;  // tone tables (with their extra entry) up to this size are read from LDS
 
 template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
 struct MainGeom {
@@ -858,7 +866,6 @@ struct MainCtx {
   QuadConst qc;
   float gains[3];
   const float* grid;
-  float lut_top;
   int n1;
   int bins;
   int copies;  // interleaved histogram copies: 4 for 256 bins
@@ -928,12 +935,11 @@ __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains
   if (c.want_hist) {
     for (int i = tid; i < 4 * kMaxHistBins; i += kBlock) fl.hist[i] = 0U;
   }
-  c.lut_lds = f.lut_size <= kLdsLut;
+  c.lut_lds = f.lut_size + 1 <= kLdsLut;
   if (c.lut_lds) {
-    for (int i = tid; i < f.lut_size; i += kBlock) fl.lut[i] = f.lut[i];
+    for (int i = tid; i <= f.lut_size; i += kBlock) fl.lut[i] = f.lut[i];
   }
   c.n1 = f.lut_size - 1;
-  c.lut_top = f.lut[c.n1];
   c.lsc = f.lsc != nullptr;
   c.grid = stage_grid(f, fl.grid);
 }
@@ -1247,13 +1253,13 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) t[i][k] = tone(fl.lut, c.n1, c.lut_top, lin[i][k]);
+        for (int k = 0; k < 3; ++k) t[i][k] = tone((const LdsFloat*)fl.lut, c.n1, lin[i][k]);
       }
     } else {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) t[i][k] = tone(f.lut, c.n1, c.lut_top, lin[i][k]);
+        for (int k = 0; k < 3; ++k) t[i][k] = tone(f.lut, c.n1, lin[i][k]);
       }
     }
     if (c.want_stats) {

@@ -41,7 +41,8 @@ isp_status isp_create(const isp_params* params, uint8_t backend, isp_pipeline** 
     check(hipGetDevice(&p->device), "hipGetDevice");
     check(hipDeviceGetAttribute(&p->launch.cu_count, hipDeviceAttributeMultiprocessorCount, p->device),
           "hipDeviceGetAttribute");
-    check(hipMalloc(&p->d_lut, p->lut.size() * sizeof(float)), "hipMalloc");
+    // The table and one entry more, below its last (see tone() in isp_kernels.hip).
+    check(hipMalloc(&p->d_lut, (p->lut.size() + 1) * sizeof(float)), "hipMalloc");
     if (p->params.c.has_lens_shading) {
       check(hipMalloc(&p->d_lsc, p->params.lsc.size() * sizeof(float)), "hipMalloc");
       upload(p->params.lsc.data(), p->params.lsc.size() * sizeof(float), p->d_lsc);
@@ -49,7 +50,9 @@ isp_status isp_create(const isp_params* params, uint8_t backend, isp_pipeline** 
     check(hipMalloc(&p->d_scratch, sizeof(DeviceScratch)), "hipMalloc");
     check(hipMemset(p->d_scratch, 0, sizeof(DeviceScratch)), "hipMemset");
     check(hipEventCreateWithFlags(&p->done, hipEventDisableTiming), "hipEventCreate");
-    upload(p->lut.data(), p->lut.size() * sizeof(float), p->d_lut);
+    std::vector<float> padded(p->lut);
+    padded.push_back(p->lut.back() - 1.0F);
+    upload(padded.data(), padded.size() * sizeof(float), p->d_lut);
     check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     p->reset();
     *out = p.release();
