@@ -43,6 +43,8 @@ def parse():
                     help="timed launches of each 4K colour conversion after the ISP (0: skip)")
     ap.add_argument("--config1-frames", type=int, default=500,
                     help="timed frames of the 1080p u16 workload (0: skip)")
+    ap.add_argument("--reference-iterations", type=int, default=100,
+                    help="timed frames of each of the reference benchmark's eight cases (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times): the reference "
                          "point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -524,6 +526,64 @@ def config1_frames(frames, cpu_frames):
     return result
 
 
+# The reference's own benchmark (modules/isp/tests/benchmark.cpp:89-160) and its published
+# RTX 5090 times, ms per frame (modules/isp/README.md:285-292).
+REFERENCE_CASES_RTX5090_MS = {
+    ("1920x1080", "bilinear", "plain"): 0.017, ("1920x1080", "bilinear", "full"): 0.100,
+    ("1920x1080", "malvar", "plain"): 0.019, ("1920x1080", "malvar", "full"): 0.093,
+    ("3840x2160", "bilinear", "plain"): 0.067, ("3840x2160", "bilinear", "full"): 0.242,
+    ("3840x2160", "malvar", "plain"): 0.074, ("3840x2160", "malvar", "full"): 0.251,
+}
+
+
+def reference_benchmark(iterations):
+    """The reference's isp_benchmark cases on this library: the demo camera's default raw
+    (12-bit u16 RGGB, vignetting 0.3) rendered on the device; "plain" = black level, manual
+    white balance, demosaic, colour matrix, sRGB; "full" adds lens shading, gray world,
+    sharpening 0.5, histograms and statistics; RGB8 out; 5 warm-up frames, then the mean of
+    `iterations` frames between two HIP events on one stream, as benchmark.cpp:136-143 times
+    them. The reference's published RTX 5090 times stand beside each."""
+    import torch
+    import holoscan_isp as isp
+    from holoscan_isp import _abi
+    stream = torch.cuda.current_stream()
+    out = {"workload": "the reference benchmark's cases (modules/isp/tests/benchmark.cpp)", "iterations": iterations,
+           "rtx5090_source": "modules/isp/README.md:285-292"}
+    for w, h in ((1920, 1080), (3840, 2160)):
+        cfg = isp.synthetic_config(width=w, height=h, vignetting=0.3)
+        raw = torch.empty(isp.lib().isp_raw_row_bytes(cfg.format, w) * h + 2 * h, dtype=torch.uint8, device="cuda")
+        pitch = isp.render_synthetic_device(cfg, raw, None, stream.cuda_stream)
+        rgb = torch.empty(3 * w * h, dtype=torch.uint8, device="cuda")
+        hist = torch.empty(1024, dtype=torch.int32, device="cuda")
+        stats = torch.empty(12, dtype=torch.float32, device="cuda")
+        view = _abi.RawView(raw.data_ptr(), w, h, pitch, cfg.format, cfg.bit_depth, cfg.pattern)
+        for method, mname in ((_abi.BILINEAR, "bilinear"), (_abi.MALVAR, "malvar")):
+            for full in (False, True):
+                params = isp.synthetic_params(cfg)
+                params.demosaic = method
+                if full:
+                    params.white_balance, params.sharpen = _abi.WB_GRAY_WORLD, 0.5
+                else:
+                    params.lens_shading = None
+                pipe = isp.IspPipeline(params)
+                h_out, s_out = (hist, stats) if full else (None, None)
+                for _ in range(5):
+                    pipe.process(view, rgb, 3 * w, _abi.OUT_RGB8, h_out, s_out, (0, 0.0), stream.cuda_stream)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(iterations):
+                    pipe.process(view, rgb, 3 * w, _abi.OUT_RGB8, h_out, s_out, (0, 0.0), stream.cuda_stream)
+                e1.record(stream)
+                e1.synchronize()
+                ms = e0.elapsed_time(e1) / iterations
+                pipe.close()
+                key = (f"{w}x{h}", mname, "full" if full else "plain")
+                ref = REFERENCE_CASES_RTX5090_MS[key]
+                out["_".join(key)] = {"ms": round(ms, 4), "gpix_s": round(w * h / ms / 1e6, 2),
+                                      "rtx5090_ms": ref, "speedup_vs_rtx5090": round(ref / ms, 2)}
+    return out
+
+
 def color_conversion(launches, cpu=True):
     """The conversion after the ISP (isp_color_convert_device) on a 4K frame: each kernel's
     average launch time from HIP events over `launches` launches, and its rate against the
@@ -690,6 +750,8 @@ def main():
         result["served_32_cameras"] = served_report(args.served_seconds)
     if args.source_launches > 0:
         result["device_source_and_check"] = source_and_check(args.source_launches, cpu=not args.no_cpu)
+    if args.reference_iterations > 0:
+        result["reference_benchmark_cases"] = reference_benchmark(args.reference_iterations)
     if args.config1_frames > 0:
         result["config1_1080p_u16"] = config1_frames(args.config1_frames, 0 if args.no_cpu else 10)
     if args.color_launches > 0:
