@@ -793,12 +793,10 @@ __device__ __forceinline__ void assign_rgb(int ch, float c, float first, float s
 // i = n1, since x <= 1), t = +0 and b - a < 0, so a + t * (b - a) is lut[n1]
 // plus -0, which is lut[n1] exactly, -0 included, as the reference returns it;
 // below, the entries read are the reference's. No clamp of the index, no
-// select. T is the table's type: an LDS table is read through an LDS pointer,
-// so its two entries are a ds_read2_b32 (through a generic pointer the
-// compiler merges the LDS and HBM paths into flat loads, and an 8-byte flat
-// load of an odd entry of LDS is slow).
-template <typename T>
-__device__ __forceinline__ float tone(T* lut, int n1, float x) {
+// select. The table is read from HBM through the caches: its 16 KB stay in L1
+// and L2, and staging it into each block's LDS measured slower (the copy at
+// every block's start, and the random reads' bank conflicts in LDS).
+__device__ __forceinline__ float tone(const float* lut, int n1, float x) {
   const float f = x * static_cast<float>(n1);
   const int i = static_cast<int>(f);
   const float t = f - static_cast<float>(i);
@@ -806,11 +804,6 @@ __device__ __forceinline__ float tone(T* lut, int n1, float x) {
   const float b = lut[i + 1];
   return a + t * (b - a);
 }
-
-// A float in LDS, addressed as such.
-typedef __attribute__((address_space(3))) const float LdsFloat;
-
-constexpr int kLdsLut = 4100;  // tone tables (with their extra entry) up to this size are read from LDS
 
 template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
 struct MainGeom {
@@ -844,7 +837,6 @@ struct MainGeom {
 // LDS of the main pass: per frame (tone table, histograms, lens shading grid,
 // reductions), and per tile (samples, demosaiced ring, lens shading table).
 struct MainFrameLds {
-  float lut[kLdsLut];
   unsigned int hist[4 * kMaxHistBins];
   float grid[kMaxGrid];
   unsigned long long part[8][4];  // per wave, up to 512 threads
@@ -870,7 +862,6 @@ struct MainCtx {
   int bins;
   int copies;  // interleaved histogram copies: 4 for 256 bins
   bool lsc;
-  bool lut_lds;
   bool want_hist;
   bool want_stats;
 };
@@ -934,10 +925,6 @@ __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains
   main_set_gains(f, gains, c);
   if (c.want_hist) {
     for (int i = tid; i < 4 * kMaxHistBins; i += kBlock) fl.hist[i] = 0U;
-  }
-  c.lut_lds = f.lut_size + 1 <= kLdsLut;
-  if (c.lut_lds) {
-    for (int i = tid; i <= f.lut_size; i += kBlock) fl.lut[i] = f.lut[i];
   }
   c.n1 = f.lut_size - 1;
   c.lsc = f.lsc != nullptr;
@@ -1249,18 +1236,10 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
         for (int k = 0; k < 3; ++k) sums[k] += to_fixed(lin[i][k]);
       }
     }
-    if (c.lut_lds) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 4; ++i) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) t[i][k] = tone((const LdsFloat*)fl.lut, c.n1, lin[i][k]);
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) t[i][k] = tone(f.lut, c.n1, lin[i][k]);
-      }
+      for (int k = 0; k < 3; ++k) t[i][k] = tone(f.lut, c.n1, lin[i][k]);
     }
     if (c.want_stats) {
 #pragma unroll
