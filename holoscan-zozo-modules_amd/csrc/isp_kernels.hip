@@ -362,9 +362,15 @@ constexpr int kAwbTile = kBatchAwbTile;
 // positions, then the count of complete quads.
 constexpr int kAwbAcc = 5;
 
+// The white balance pass's LDS histograms (white patch) in interleaved copies,
+// `bin * kAwbCopies + lane % kAwbCopies`, which spread the atomics of lanes
+// that share a bin (flat regions) over addresses. After the flush, the last
+// block reuses the first 3 x kAwbBins entries for the frame's totals.
+constexpr int kAwbCopies = 2;
+
 // LDS of the white balance pass: per frame, and per tile.
 struct AwbFrameLds {
-  unsigned int hist[3 * kAwbBins];
+  unsigned int hist[3 * kAwbBins * kAwbCopies];
   float grid[kMaxGrid];
   unsigned long long part[kThreads / 64][4];
   unsigned long long sums[4];
@@ -389,7 +395,7 @@ struct AwbCtx {
 __device__ __forceinline__ void awb_begin(const FrameDesc& f, AwbFrameLds& fl, AwbCtx& c) {
   c.patch = f.wb_mode == ISP_WB_WHITE_PATCH;
   if (c.patch) {
-    for (int i = threadIdx.x; i < 3 * kAwbBins; i += kThreads) fl.hist[i] = 0U;
+    for (int i = threadIdx.x; i < 3 * kAwbBins * kAwbCopies; i += kThreads) fl.hist[i] = 0U;
   }
   c.grid = stage_grid(f, fl.grid);
   const float ones[3] = {1.0F, 1.0F, 1.0F};
@@ -575,7 +581,8 @@ __device__ __forceinline__ void awb_tile(const FrameDesc& f, const AwbCtx& c, Aw
       pos[k] += to_fixed(v);
       if (c.patch) {
         const int b = static_cast<int>(v * static_cast<float>(kAwbBins));
-        atomicAdd(&fl.hist[c.colour[k] * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1)], 1U);
+        const int bin = c.colour[k] * kAwbBins + (b < kAwbBins - 1 ? b : kAwbBins - 1);
+        atomicAdd(&fl.hist[bin * kAwbCopies + (threadIdx.x & (kAwbCopies - 1))], 1U);
       }
     }
   }
@@ -627,7 +634,10 @@ __device__ __forceinline__ unsigned int awb_flush(AwbScratch* a, AwbFrameLds& fl
   if (c.patch) {
     const int x = blockIdx.x % kSlots;
     for (int i = tid; i < 3 * kAwbBins; i += kThreads) {
-      if (fl.hist[i] != 0U) token ^= atomicAdd(&a->hist[x][i / kAwbBins][i % kAwbBins], fl.hist[i]);
+      unsigned int total = 0U;
+#pragma unroll
+      for (int k = 0; k < kAwbCopies; ++k) total += fl.hist[i * kAwbCopies + k];
+      if (total != 0U) token ^= atomicAdd(&a->hist[x][i / kAwbBins][i % kAwbBins], total);
     }
   }
   return token;
