@@ -93,7 +93,9 @@ def float_edge_cases():
     - signed zeros: a random mosaic with many samples below black, a colour matrix whose
       last row is all negative (so black pixels give -0 there, which clamp01 keeps), and a
       user tone table whose first entry is -0.0 (validation accepts it as 0), so a -0 can
-      reach the RGB32F output."""
+      reach the RGB32F output;
+    - the table's top entry: x = 1 on saturated pixels, through a table whose last entry is
+      -0.0."""
     out = []
     cfg = camera(_abi.GRBG, FORMATS[2], 67, 37)
     frame = isp.render_synthetic(cfg)
@@ -118,6 +120,17 @@ def float_edge_cases():
                           ccm=(1.0, 0.0, 0.0, 0.0, 1.0, 0.0, -0.25, -0.5, -0.25),
                           tone_curve=_abi.TONE_LUT, lut=[float(v) for v in lut])
         out.append((f"signed_zero_{method}_{sharpen}", mosaic, r, _abi.OUT_RGB32F))
+    # The table's top entry: a decreasing table ending in -0.0, reached at x = 1 by a frame
+    # with a saturated band (the reference returns lut[n - 1] itself there).
+    bright = rng.integers(0, 4096, size=(29, 45), dtype=np.uint16)
+    bright[8:20, :] = 4095
+    top = isp.RawFrame(bright.view(np.uint8).ravel(), 45, 29, 90, _abi.RAW_U16, 12, _abi.GBRG)
+    falling = np.linspace(1.0, 0.0, 4097, dtype=np.float32)
+    falling[-1] = np.float32(-0.0)
+    for method, sharpen in ((_abi.BILINEAR, 0.0), (_abi.MALVAR, 0.8)):
+        r = isp.IspParams(black_level=64.0, wb_gains=(1.0, 1.0, 1.0), demosaic=method, sharpen=sharpen,
+                          tone_curve=_abi.TONE_LUT, lut=[float(v) for v in falling])
+        out.append((f"table_top_{method}_{sharpen}", top, r, _abi.OUT_RGB32F))
     return out
 
 

@@ -97,6 +97,9 @@ def test_float_edge_cases_reach_the_edges():
     frame, params, fmt = edge["signed_zero_1_0.0"]
     img = OraclePipeline(params).process(frame, fmt, True, (10_000_000, 2.0)).image.view(np.float32)
     assert np.any((img == 0) & np.signbit(img))
+    frame, params, fmt = edge["table_top_0_0.0"]
+    img = OraclePipeline(params).process(frame, fmt, True, (10_000_000, 2.0)).image.view(np.float32)
+    assert np.count_nonzero((img == 0) & np.signbit(img)) > 100  # the saturated band
 
 
 @needs_ref
