@@ -44,6 +44,10 @@ constexpr int kLatW = 2 * kSrcTileW + 2;  // lattice columns of a tile: points 2
 constexpr int kLatH = 2 * kSrcTileH + 2;
 constexpr int kLatPoints = kLatW * kLatH;
 constexpr int kCheckEntries = ISP_HISTOGRAM_CHANNELS * ISP_HISTOGRAM_BINS;
+// The block's LDS histograms in interleaved copies (`entry * kCheckCopies +
+// lane % kCheckCopies`): the chart's flat patches give many lanes one code, and
+// copies spread their atomics over addresses.
+constexpr int kCheckCopies = 8;
 
 struct alignas(16) P4 {
   float v[4];
@@ -171,7 +175,7 @@ __device__ __forceinline__ int truth_code(const float* lut, int size, float line
 
 struct CheckLds {
   P4 lat[kLatPoints];
-  unsigned int bins[kCheckEntries];
+  unsigned int bins[kCheckEntries * kCheckCopies];
   unsigned long long wave_err[kSrcThreads / 64];
   unsigned int wave_miss[kSrcThreads / 64];
   int flag;
@@ -184,7 +188,8 @@ __global__ void __launch_bounds__(kSrcThreads) check_kernel(const Chart s, const
                                                             isp_frame_check_result* result, int tiles_x, int tiles) {
   __shared__ CheckLds lds;
   const int tid = static_cast<int>(threadIdx.x);
-  for (int i = tid; i < kCheckEntries; i += kSrcThreads) lds.bins[i] = 0U;
+  for (int i = tid; i < kCheckEntries * kCheckCopies; i += kSrcThreads) lds.bins[i] = 0U;
+  const int copy = tid & (kCheckCopies - 1);
   unsigned long long err = 0ULL;
   for (int tile = static_cast<int>(blockIdx.x); tile < tiles; tile += static_cast<int>(gridDim.x)) {
     const int ty = tile / tiles_x;
@@ -212,10 +217,10 @@ __global__ void __launch_bounds__(kSrcThreads) check_kernel(const Chart s, const
         err += static_cast<unsigned long long>(d * d);
       }
       const int luma = (2126 * q[0] + 7152 * q[1] + 722 * q[2] + 5000) / 10000;
-      atomicAdd(&lds.bins[luma], 1U);
-      atomicAdd(&lds.bins[ISP_HISTOGRAM_BINS + q[0]], 1U);
-      atomicAdd(&lds.bins[2 * ISP_HISTOGRAM_BINS + q[1]], 1U);
-      atomicAdd(&lds.bins[3 * ISP_HISTOGRAM_BINS + q[2]], 1U);
+      atomicAdd(&lds.bins[luma * kCheckCopies + copy], 1U);
+      atomicAdd(&lds.bins[(ISP_HISTOGRAM_BINS + q[0]) * kCheckCopies + copy], 1U);
+      atomicAdd(&lds.bins[(2 * ISP_HISTOGRAM_BINS + q[1]) * kCheckCopies + copy], 1U);
+      atomicAdd(&lds.bins[(3 * ISP_HISTOGRAM_BINS + q[2]) * kCheckCopies + copy], 1U);
     }
   }
   // The block's share into its slot of the scratch; the returned values are
@@ -231,7 +236,9 @@ __global__ void __launch_bounds__(kSrcThreads) check_kernel(const Chart s, const
     if (block != 0ULL) token ^= fold(atomicAdd(&scratch->err[slot].v, block));
   }
   for (int i = tid; i < kCheckEntries; i += kSrcThreads) {
-    const unsigned int c = lds.bins[i];
+    unsigned int c = 0U;
+#pragma unroll
+    for (int k = 0; k < kCheckCopies; ++k) c += lds.bins[i * kCheckCopies + k];
     if (c != 0U) token ^= atomicAdd(&scratch->hist[slot][i], c);
   }
   if (!complete_grid(scratch->done, &scratch->top, token, &lds.flag)) return;
