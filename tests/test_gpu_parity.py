@@ -128,6 +128,30 @@ def test_padded_rows(torch_gpu):
         compare("padded vs oracle", b, OraclePipeline(isp.synthetic_params(config, 5)).process(padded))
 
 
+@pytest.mark.parametrize("size", [(1920, 1080), (3840, 2160)])
+def test_reference_benchmark_cases(torch_gpu, size):
+    """The reference benchmark's eight cases (modules/isp/tests/benchmark.cpp:89-160) as the
+    bench times them: the demo camera's default raw with vignetting 0.3, plain (no lens
+    shading) and full (gray world, sharpening 0.5, histograms and statistics), bilinear and
+    Malvar, RGB8; three frames each, so the gray-world state runs too."""
+    w, h = size
+    cfg = isp.synthetic_config(width=w, height=h, vignetting=0.3)
+    frame = isp.render_synthetic(cfg)
+    for method in (_abi.BILINEAR, _abi.MALVAR):
+        for full in (False, True):
+            print(f"reference case {w}x{h} method {method} full {full}", flush=True)
+            params = isp.synthetic_params(cfg)
+            params.demosaic = method
+            if full:
+                params.white_balance, params.sharpen = _abi.WB_GRAY_WORLD, 0.5
+            else:
+                params.lens_shading = None
+            gpu, cpu = isp.IspPipeline(params), OraclePipeline(params)
+            for _ in range(3):
+                compare(f"{w}x{h} {method} {full}", gs.run(gpu, frame, _abi.OUT_RGB8, full, (0, 0.0)),
+                        cpu.process(frame, _abi.OUT_RGB8, full, (0, 0.0), threads=8))
+
+
 @pytest.mark.parametrize("which", ["config1", "config2", "config5"])
 def test_baseline_configs_reduced(torch_gpu, which):
     """The benchmark configurations' pipelines on their seeded noisy scenes at reduced sizes."""
