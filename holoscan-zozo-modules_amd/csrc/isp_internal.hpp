@@ -45,11 +45,15 @@ void set_error(const std::string& message);
 void clear_error();
 
 // A camera's exposure carried on the device from one frame to the next (served
-// form): the next frame runs at the previous frame's suggestion.
+// form): the next frame runs at the previous frame's suggestion. Every pipeline
+// holds one from creation to destruction, so a frame table built earlier (the
+// batched form) never points at freed memory; `enabled` switches the feedback,
+// and the kernels read it when the frame runs.
 struct AeState {
   long long exposure_ns;
   float gain;
-  int valid;  // 0: use the frame's own exposure
+  int valid;    // 0: use the frame's own exposure
+  int enabled;  // 0: no feedback; the frame's own exposure, no state written
 };
 
 // Everything one frame's kernels read, passed by value (FrameParams,
@@ -89,8 +93,9 @@ struct FrameDesc {
   int wb_mode;
   float percentile;
   float smoothing;
-  // Optional device exposure state: read instead of exposure_ns / exposure_gain
-  // once valid, and replaced by this frame's suggestion.
+  // The pipeline's device exposure state: while enabled, read instead of
+  // exposure_ns / exposure_gain once valid, and replaced by this frame's
+  // suggestion.
   AeState* ae_state;
 };
 
@@ -191,6 +196,8 @@ struct LaunchConfig {
 void launch_awb(const FrameDesc& f, DeviceScratch* scratch, const LaunchConfig& cfg, void* stream);
 void launch_main(const FrameDesc& f, DeviceScratch* scratch, bool malvar, bool sharpen,
                  const LaunchConfig& cfg, void* stream);
+// The statistics epilogue over `count` cases in device memory (verification).
+void launch_epilogue(const isp_epilogue_case* cases, float* out, int count, void* stream);
 // The batch kernels' resident blocks per CU and their launches (grid blocks);
 // the main-pass kernel for batches with sharpened frames differs from the one
 // without.

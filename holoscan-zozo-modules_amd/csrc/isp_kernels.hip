@@ -936,7 +936,7 @@ template <int kBlock = kThreads>
 __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains[3], MainFrameLds& fl, MainCtx& c) {
   const int tid = threadIdx.x;
   c.want_hist = f.histogram != nullptr;
-  c.want_stats = f.stats != nullptr || f.ae_state != nullptr;
+  c.want_stats = f.stats != nullptr || (f.ae_state != nullptr && f.ae_state->enabled != 0);
   c.bins = f.bins;
   c.copies = kHistEntries / (4 * c.bins);
   main_set_gains(f, gains, c);
@@ -1446,7 +1446,7 @@ __device__ __forceinline__ void main_finish(const FrameDesc& f, MainScratch* m, 
     const unsigned long long sums[4] = {fl.sums[0], fl.sums[1], fl.sums[2], fl.sums[3]};
     long long exposure_ns = f.exposure_ns;
     float exposure_gain = f.exposure_gain;
-    AeState* ae = f.ae_state;
+    AeState* ae = f.ae_state != nullptr && f.ae_state->enabled != 0 ? f.ae_state : nullptr;
     if (ae != nullptr && ae->valid != 0) {
       exposure_ns = ae->exposure_ns;
       exposure_gain = ae->gain;
@@ -1813,6 +1813,25 @@ batch_main_kernel_plain_wide(const BatchTable* __restrict__ table, BatchControl*
   batch_main_body<false, kWideBlock>(table, ctl);
 }
 
+// ---- verification: the statistics epilogue on the device ---------------------
+
+// finish_stats (isp_math.hpp:387-425) over many cases, one per thread: the same
+// function main_finish runs, so its double arithmetic (the double-double
+// log2 / exp2 of isp_ae.hpp) is checked as the device evaluates it.
+__global__ void __launch_bounds__(kThreads) epilogue_kernel(const isp_epilogue_case* __restrict__ cases,
+                                                            float* __restrict__ out, int count) {
+  const int i = static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
+  if (i >= count) return;
+  const isp_epilogue_case& c = cases[i];
+  const unsigned long long sums[4] = {c.sums[0], c.sums[1], c.sums[2], c.sums[3]};
+  const AeParams ae{c.ae.target, c.ae.damping, c.ae.min_exposure_ns, c.ae.max_exposure_ns, c.ae.min_gain,
+                    c.ae.max_gain};
+  float r[ISP_STATS_SIZE];
+  finish_stats(sums, c.pixels, c.gains, ae, c.exposure.exposure_ns, c.exposure.gain, r);
+#pragma unroll
+  for (int k = 0; k < ISP_STATS_SIZE; ++k) out[static_cast<long long>(i) * ISP_STATS_SIZE + k] = r[k];
+}
+
 // ---- launches ---------------------------------------------------------------
 
 void check(hipError_t status, const char* what) {
@@ -1925,6 +1944,13 @@ void launch_main(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen
       launch_main_f<ISP_RAW_RAW12>(f, s, malvar, sharpen, cfg, st);
       break;
   }
+}
+
+void launch_epilogue(const isp_epilogue_case* cases, float* out, int count, void* stream) {
+  const int grid = (count + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(epilogue_kernel, dim3(grid), dim3(kThreads), 0, static_cast<hipStream_t>(stream), cases, out,
+                     count);
+  check(hipGetLastError(), "epilogue_kernel launch");
 }
 
 int batch_main_blocks_per_cu(bool sharpen, bool wide) {

@@ -49,6 +49,10 @@ isp_status isp_create(const isp_params* params, uint8_t backend, isp_pipeline** 
     }
     check(hipMalloc(&p->d_scratch, sizeof(DeviceScratch)), "hipMalloc");
     check(hipMemset(p->d_scratch, 0, sizeof(DeviceScratch)), "hipMemset");
+    // The exposure state lives as long as the pipeline (a batch table built
+    // earlier keeps its address); feedback starts switched off.
+    check(hipMalloc(&p->d_ae, sizeof(AeState)), "hipMalloc");
+    check(hipMemset(p->d_ae, 0, sizeof(AeState)), "hipMemset");
     check(hipEventCreateWithFlags(&p->done, hipEventDisableTiming), "hipEventCreate");
     std::vector<float> padded(p->lut);
     padded.push_back(p->lut.back() - 1.0F);
@@ -107,33 +111,41 @@ isp_status isp_process(isp_pipeline* pipeline, const isp_raw_view* raw, const is
 isp_status isp_set_exposure_feedback(isp_pipeline* pipeline, int32_t enable, const isp_frame_exposure* initial) {
   return guarded([&] {
     if (pipeline == nullptr) throw invalid_argument("pipeline is null");
+    if (enable != 0) {
+      if (initial == nullptr || initial->exposure_ns <= 0) {
+        throw invalid_argument("exposure feedback needs an initial exposure_ns > 0");
+      }
+      if (!(initial->gain >= 0.0F) || !std::isfinite(initial->gain)) {
+        throw invalid_argument("exposure feedback needs a finite initial gain >= 0");
+      }
+    }
+    // Queued frames finish with the state they were queued with.
     if (pipeline->queued) check(hipEventSynchronize(pipeline->done), "hipEventSynchronize");
-    if (enable == 0) {
-      if (pipeline->d_ae != nullptr) check(hipFree(pipeline->d_ae), "hipFree");
-      pipeline->d_ae = nullptr;
-      return;
-    }
-    if (initial == nullptr || initial->exposure_ns <= 0) {
-      throw invalid_argument("exposure feedback needs an initial exposure_ns > 0");
-    }
-    if (!(initial->gain >= 0.0F) || !std::isfinite(initial->gain)) {
-      throw invalid_argument("exposure feedback needs a finite initial gain >= 0");
-    }
-    if (pipeline->d_ae == nullptr) check(hipMalloc(&pipeline->d_ae, sizeof(AeState)), "hipMalloc");
-    const AeState state{initial->exposure_ns, initial->gain, 1};
+    const AeState state = enable != 0 ? AeState{initial->exposure_ns, initial->gain, 1, 1} : AeState{0, 0.0F, 0, 0};
     upload(&state, sizeof(state), pipeline->d_ae);
+    pipeline->ae_enabled = enable != 0;
   });
 }
 
 isp_status isp_get_exposure_state(isp_pipeline* pipeline, isp_frame_exposure* state) {
   return guarded([&] {
     if (pipeline == nullptr || state == nullptr) throw invalid_argument("pipeline or state is null");
-    if (pipeline->d_ae == nullptr) throw invalid_argument("exposure feedback is not enabled");
+    if (!pipeline->ae_enabled) throw invalid_argument("exposure feedback is not enabled");
     if (pipeline->queued) check(hipEventSynchronize(pipeline->done), "hipEventSynchronize");
     AeState host{};
     check(hipMemcpy(&host, pipeline->d_ae, sizeof(host), hipMemcpyDeviceToHost), "hipMemcpy");
     state->exposure_ns = host.exposure_ns;
     state->gain = host.gain;
+  });
+}
+
+isp_status isp_epilogue_finish_stats_device(const isp_epilogue_case* cases, float* out, int32_t count,
+                                            isp_stream_t stream) {
+  return guarded([&] {
+    if (count < 0) throw invalid_argument("count must be >= 0");
+    if (count == 0) return;
+    if (cases == nullptr || out == nullptr) throw invalid_argument("cases or out is null");
+    launch_epilogue(cases, out, count, stream);
   });
 }
 

@@ -81,7 +81,8 @@ struct isp_pipeline {
   float* d_lut = nullptr;
   float* d_lsc = nullptr;
   DeviceScratch* d_scratch = nullptr;
-  isp_amd::AeState* d_ae = nullptr;  // device exposure feedback, when enabled
+  isp_amd::AeState* d_ae = nullptr;  // device exposure feedback state, for the pipeline's lifetime
+  bool ae_enabled = false;           // the host's copy of d_ae->enabled
   hipEvent_t done = nullptr;
   bool queued = false;
   // Optional kernel timing: events around each launch, on the launch stream.

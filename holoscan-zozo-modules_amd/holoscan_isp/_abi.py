@@ -77,6 +77,11 @@ class FrameExposure(C.Structure):
     _fields_ = [("exposure_ns", C.c_int64), ("gain", C.c_float)]
 
 
+class EpilogueCase(C.Structure):
+    _fields_ = [("sums", C.c_uint64 * 4), ("pixels", C.c_int64), ("gains", C.c_float * 3),
+                ("ae", AutoExposure), ("exposure", FrameExposure)]
+
+
 class BatchFrame(C.Structure):
     _fields_ = [("pipeline", C.c_void_p), ("raw", RawView), ("out", RgbView),
                 ("stats", StatsOut), ("exposure", FrameExposure)]
@@ -233,6 +238,7 @@ PROTOTYPES = [
     ("isp_epilogue_exp2", C.c_double, [C.c_double]),
     ("isp_epilogue_finish_stats", None, [P(C.c_uint64), C.c_int64, P(C.c_float), P(AutoExposure),
                                          P(FrameExposure), P(C.c_float)]),
+    ("isp_epilogue_finish_stats_device", C.c_int, [_void, _void, C.c_int32, _void]),
     ("isp_create", C.c_int, [P(Params), C.c_uint8, P(_void)]),
     ("isp_destroy", C.c_int, [_void]),
     ("isp_backend", C.c_uint8, [_void]),

@@ -196,6 +196,22 @@ double isp_epilogue_exp2(double y);
 void isp_epilogue_finish_stats(const uint64_t sums[4], int64_t pixels, const float gains[3],
                                const isp_auto_exposure* ae, const isp_frame_exposure* exposure,
                                float* out);
+/* One case of the statistics epilogue: the frame's four fixed-point sums (R, G,
+ * B, saturated samples), its pixel count, white balance gains, auto-exposure
+ * settings and capture exposure. */
+typedef struct isp_epilogue_case {
+  uint64_t sums[4];
+  int64_t pixels;
+  float gains[3];
+  isp_auto_exposure ae;
+  isp_frame_exposure exposure;
+} isp_epilogue_case;
+/* Verification (no reference counterpart): the same epilogue evaluated on the
+ * device, by the code the main pass's last block runs, one case per thread:
+ * `cases` (count entries) and `out` (count x ISP_STATS_SIZE floats) are device
+ * memory; queued on `stream`. */
+isp_status isp_epilogue_finish_stats_device(const isp_epilogue_case* cases, float* out, int32_t count,
+                                            isp_stream_t stream);
 
 /* ---- the pipeline (isp_pipeline.hpp:39-74) ------------------------------ */
 

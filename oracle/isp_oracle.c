@@ -396,6 +396,15 @@ void oracle_finish_stats(const uint64_t sums[4], int64_t pixels, const float gai
   out[ISP_STAT_SATURATED_FRACTION] = (float)((double)sums[3] / (double)pixels);
 }
 
+/* oracle_finish_stats over `count` cases (isp_epilogue_case, include/isp_c.h):
+ * out holds ISP_STATS_SIZE floats per case. */
+void oracle_finish_stats_cases(const isp_epilogue_case* cases, float* out, int count) {
+  for (int i = 0; i < count; ++i) {
+    const isp_epilogue_case* c = &cases[i];
+    oracle_finish_stats(c->sums, c->pixels, c->gains, &c->ae, &c->exposure, out + (size_t)i * ISP_STATS_SIZE);
+  }
+}
+
 /* tone_lut (isp_params.cpp:166-182). Returns the size, writes min(size, capacity). */
 int oracle_tone_lut(const isp_params* params, float* out, int capacity) {
   if (params->tone_curve == ISP_TONE_LUT) {

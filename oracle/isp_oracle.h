@@ -46,6 +46,8 @@ int oracle_load_raw(const isp_raw_view* raw, int x, int y);
 void oracle_finish_stats(const uint64_t sums[4], int64_t pixels, const float gains[3],
                          const isp_auto_exposure* ae, const isp_frame_exposure* exposure,
                          float* out);
+/* oracle_finish_stats over `count` cases; out holds ISP_STATS_SIZE floats per case. */
+void oracle_finish_stats_cases(const isp_epilogue_case* cases, float* out, int count);
 /* histogram_percentile (isp_math.hpp:336-352) over a 1024-bin histogram. */
 float oracle_histogram_percentile(const uint32_t* hist, float percentile);
 

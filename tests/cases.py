@@ -220,6 +220,35 @@ def config3(divisor=1, seed=300):
     return out
 
 
+# Lens shading grids past the 1,024 values a block stages in LDS (validation accepts up to
+# 256 x 256 x 4, isp_params.cpp:102-112): the kernels then read the grid from device memory,
+# through the tile's table where a tile's rows span at most two grid rows and per sample where
+# they span more. Rows: grid width, height, channels; frame width, height; raw format, bits,
+# pattern; demosaic, sharpening, white balance.
+LARGE_GRIDS = [
+    (17, 17, 4, 320, 240, _abi.RAW_RAW12, 12, _abi.GRBG, _abi.MALVAR, 0.5, _abi.WB_GRAY_WORLD),
+    (41, 25, 1, 300, 200, _abi.RAW_RAW10, 10, _abi.RGGB, _abi.BILINEAR, 0.0, _abi.WB_WHITE_PATCH),
+    (32, 32, 1, 300, 200, _abi.RAW_U16, 12, _abi.BGGR, _abi.MALVAR, 0.0, _abi.WB_GRAY_WORLD),
+    (64, 48, 4, 400, 300, _abi.RAW_U16, 14, _abi.GBRG, _abi.MALVAR, 0.7, _abi.WB_WHITE_PATCH),
+    (64, 64, 4, 1920, 1080, _abi.RAW_RAW10, 10, _abi.BGGR, _abi.BILINEAR, 0.0, _abi.WB_GRAY_WORLD),
+    (256, 256, 1, 640, 480, _abi.RAW_RAW12, 12, _abi.RGGB, _abi.MALVAR, 0.5, _abi.WB_GRAY_WORLD),
+    (256, 256, 4, 512, 256, _abi.RAW_U8, 8, _abi.RGGB, _abi.MALVAR, 1.0, _abi.WB_WHITE_PATCH),
+]
+BLACK_OF_BITS = {8: 16.0, 10: 64.0, 12: 240.0, 14: 1024.0, 16: 4096.0}
+
+
+def large_grid(row, seed):
+    """(scene, params, out_format) of a LARGE_GRIDS row: seeded gains uniform in [0.5, 2.5)."""
+    gw, gh, ch, w, h, fmt, bits, pattern, dem, sharpen, wb = row
+    black = BLACK_OF_BITS[bits]
+    sc = baseline_scene(w, h, fmt, bits, pattern, black, seed)
+    gains = np.random.default_rng(seed).uniform(0.5, 2.5, gw * gh * ch).astype(np.float32)
+    p = isp.IspParams(black_level=black, lens_shading=isp.LensShading(gw, gh, ch, [float(g) for g in gains]),
+                      white_balance=wb, wb_gains=(1.7, 1.0, 1.4), demosaic=dem, sharpen=sharpen, ccm=BASELINE_CCM,
+                      tone_curve=_abi.TONE_SRGB)
+    return sc, p, _abi.OUT_RGBA8
+
+
 CONFIG4_SIZES = [(1280, 720), (1920, 1080), (1920, 1200), (3840, 2160)]
 
 

@@ -29,6 +29,7 @@ ORACLE_PROTOTYPES = [
     ("oracle_load_raw", C.c_int, [P(_abi.RawView), C.c_int, C.c_int]),
     ("oracle_finish_stats", None, [P(C.c_uint64), C.c_int64, P(C.c_float), P(_abi.AutoExposure),
                                    P(_abi.FrameExposure), P(C.c_float)]),
+    ("oracle_finish_stats_cases", None, [_v, _v, C.c_int]),
     ("oracle_histogram_percentile", C.c_float, [P(C.c_uint32), C.c_float]),
     ("oracle_frame_check", None, [P(C.c_float), C.c_int, C.c_int, P(C.c_float), C.c_int, _v, C.c_size_t,
                                   P(C.c_uint32), P(C.c_float), P(_abi.FrameCheckResult)]),

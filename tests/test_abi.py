@@ -45,7 +45,8 @@ STRUCTS = {"isp_params": _abi.Params, "isp_lens_shading": _abi.LensShading,
            "isp_server_result": _abi.ServerResult, "isp_server_counters": _abi.ServerCounters,
            "isp_tensor": _abi.Tensor, "isp_image": _abi.Image, "isp_sample_metadata": _abi.SampleMetadata,
            "isp_op_config": _abi.OpConfig, "isp_op_port": _abi.OpPort, "isp_op_statistics_t": _abi.OpStatistics,
-           "isp_op_runtime": _abi.OpRuntime, "isp_frame_check_result": _abi.FrameCheckResult}
+           "isp_op_runtime": _abi.OpRuntime, "isp_frame_check_result": _abi.FrameCheckResult,
+           "isp_epilogue_case": _abi.EpilogueCase}
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):

@@ -226,6 +226,51 @@ def test_device_exposure_feedback_alone_and_batched(torch_gpu):
             exposure = (int(np.floor(np.float64(e) + 0.5)), float(r.stats[_abi.STAT_SUGGESTED_GAIN]))
 
 
+def test_large_lens_shading_grids_batched(torch_gpu):
+    """The large lens shading grids (device-memory reads, tile tables and per-sample gains) in one
+    batch: the batch kernels against the oracle over two ticks."""
+    specs = [cases.large_grid(row, 700 + i) for i, row in enumerate(cases.LARGE_GRIDS)]
+    batched = run_batch(specs, ticks=2)
+    oracle = run_oracle(specs, ticks=2)
+    for t in range(2):
+        for i in range(len(specs)):
+            compare(f"tick {t} grid {cases.LARGE_GRIDS[i][:3]}", batched[t][i], oracle[t][i])
+
+
+def test_feedback_switched_after_the_frames_are_bound(torch_gpu):
+    """Exposure feedback switched on, off and on again between isp_batch_set_frames and the
+    launches: each launch runs with the switch as it stands then (the bound table keeps the
+    pipelines' exposure state, which lives as long as the pipeline), equal to per-frame processing
+    with the same switches."""
+    import torch
+    specs = mixed_specs()[:4]
+    slots = [Slot(*s) for s in specs]
+    pipes = [isp.IspPipeline(s.params) for s in slots]
+    singles = [isp.IspPipeline(s.params) for s in slots]
+    batch = isp.IspBatch(make_schedule(graph=0))
+    stream = torch.cuda.current_stream().cuda_stream
+    batch.set_frames([s.entry(p) for s, p in zip(slots, pipes)])
+    switches = [(8_000_000, 1.5), "keep", None, (20_000_000, 3.0), "keep", None]
+    for t, switch in enumerate(switches):
+        if switch != "keep":
+            batch.synchronize()
+            for p in pipes + singles:
+                p.set_exposure_feedback(switch)
+        for s in slots:
+            s.load(t)
+        batch.launch(stream)
+        torch.cuda.synchronize()
+        got = [s.result() for s in slots]
+        for i, (s, p) in enumerate(zip(slots, singles)):
+            want = gs.run(p, s.frame, s.out_format, True, EXPOSURE, d_raw=s.d_raw)
+            compare(f"tick {t} stream {i}", got[i], want)
+        if switch is None:
+            with pytest.raises(ValueError, match="not enabled"):
+                pipes[0].exposure_state()
+        else:
+            assert pipes[0].exposure_state() == singles[0].exposure_state()
+
+
 def test_autotuner_cache_and_state(torch_gpu, tmp_path):
     """The tuner returns a valid schedule, stores it, reuses it without timing, and leaves the
     streams' white balance state as it found it."""

@@ -96,6 +96,18 @@ def test_float_edges(torch_gpu, case):
         compare(name, gs.run(gpu, frame, out_format, True, EXPOSURE), cpu.process(frame, out_format, True, EXPOSURE))
 
 
+@pytest.mark.parametrize("row", cases.LARGE_GRIDS, ids=lambda r: f"{r[0]}x{r[1]}x{r[2]} on {r[3]}x{r[4]}")
+def test_large_lens_shading_grids(torch_gpu, row):
+    """Grids of more than 1,024 values (read from device memory, through the tile tables and per
+    sample) and the 1,024-value boundary, in both passes (white balance and main), two frames."""
+    sc, params, out_format = cases.large_grid(row, 600 + row[0] + row[3])
+    frame = isp.render_scene(sc)
+    gpu, cpu = isp.IspPipeline(params), OraclePipeline(params)
+    for _ in range(2):
+        compare(f"grid {row[:3]}", gs.run(gpu, frame, out_format, True, EXPOSURE),
+                cpu.process(frame, out_format, True, EXPOSURE, threads=8))
+
+
 def test_random_mosaics(torch_gpu):
     """LCG mosaics (cpu_reference_test.cpp:57-66) at odd sizes, every option on."""
     rng = np.random.default_rng(12345)

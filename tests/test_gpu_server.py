@@ -1,10 +1,13 @@
 """The served form (isp_server_*): frames of many cameras submitted as they
 arrive, admitted against an HBM budget and run in continuous batches by the
-native scheduling thread. Each camera's results must be the same, bit for bit
-(images, histograms, statistics, and the device exposure state that feeds each
-frame from its predecessor), alone and in the mix, whatever the budget, the
-launch slots, the batch size and whether frames come from host or device
-memory."""
+native scheduling thread. Each camera's results (images, histograms,
+statistics) must be the CPU oracle's bit for bit, with the device exposure
+state feeding each frame from its predecessor's suggestion as the oracle chain
+does it by hand (as the reference checks each frame against its CPU backend,
+modules/isp/tests/pipeline_test.cpp:164-235), alone and in the mix, whatever
+the budget, the launch slots, the batch size and whether frames come from host
+or device memory; and at the served workload's own size: 32 cameras of 720p to
+4K RAW10 / RAW12 with white patch balance."""
 import random
 
 import numpy as np
@@ -16,6 +19,7 @@ from holoscan_isp.params import output_channel_bytes, output_channels
 
 import cases
 import gpu_support as gs
+from oracle_lib import OraclePipeline
 
 pytestmark = pytest.mark.gpu
 
@@ -119,6 +123,24 @@ def alone(specs, n):
     return want
 
 
+def oracle_chain(specs, n, threads=16):
+    """Each camera through the CPU oracle, frame after frame, its exposure carried by hand as the
+    device state carries it: the first frame at INITIAL, each later one at its predecessor's
+    suggestion, exposure_ns = (long long)(double(stats[6]) + 0.5) and gain = stats[7], whenever
+    stats[6] > 0."""
+    want = {}
+    for c, (sc, p, of) in enumerate(specs):
+        cpu = OraclePipeline(p)
+        exposure = INITIAL
+        for k, f in enumerate(frames_of(sc, n)):
+            h = cpu.process(f, of, True, exposure, threads=threads)
+            want[(c, k)] = h
+            suggested = float(np.float64(h.stats[_abi.STAT_SUGGESTED_EXPOSURE_NS]))
+            if suggested > 0.0:
+                exposure = (int(suggested + 0.5), float(h.stats[_abi.STAT_SUGGESTED_GAIN]))
+    return want
+
+
 def compare(got, want):
     assert got.keys() == want.keys()
     for key in sorted(want):
@@ -132,9 +154,25 @@ def compare(got, want):
 def test_same_alone_and_in_the_mix(torch_gpu):
     specs = cameras()
     got, counters = serve(specs, 3)
+    compare(got, oracle_chain(specs, 3))
     compare(got, alone(specs, 3))
     assert counters.completed == 3 * len(specs)
     assert counters.budget_used == 0
+
+
+def test_served_workload_full_size_matches_the_oracle(torch_gpu):
+    """The 32-camera served mix at its own sizes (720p, 1080p, 1920x1200 and 4K; RAW10 and RAW12;
+    white patch at the 0.99 percentile; device exposure feedback), three frames a camera
+    submitted at once in a shuffled order, so that the loop admits them in batches under a 1 GiB
+    budget: every image, histogram and statistics vector equals the oracle's, frame by frame."""
+    specs = cases.config4()
+    got, counters = serve(specs, 3, budget=1 << 30)
+    assert counters.completed == 3 * len(specs)
+    assert counters.budget_peak <= 1 << 30
+    assert counters.launches < counters.completed  # frames shared launches
+    for c in range(len(specs)):
+        want = oracle_chain(specs[c:c + 1], 3)
+        compare({(0, k): got.pop((c, k)) for k in range(3)}, want)
 
 
 def test_tight_budget_admits_in_turn(torch_gpu):
@@ -143,7 +181,7 @@ def test_tight_budget_admits_in_turn(torch_gpu):
     specs = cameras()
     budget = 2 * (512 * 288 * 4 + 512 * 288 * 3 // 2) + (1 << 16)
     got, counters = serve(specs, 2, budget=budget)
-    compare(got, alone(specs, 2))
+    compare(got, oracle_chain(specs, 2))
     assert counters.budget_peak <= budget
     assert counters.launches >= 4
 
@@ -154,7 +192,7 @@ def test_tight_budget_admits_in_turn(torch_gpu):
 def test_modes_give_the_same_bits(torch_gpu, options):
     specs = cameras()[:4]
     got, _ = serve(specs, 2, **options)
-    compare(got, alone(specs, 2))
+    compare(got, oracle_chain(specs, 2))
 
 
 def test_admission_errors(torch_gpu):

@@ -1,11 +1,12 @@
 """The kernels' gfx950 assembly keeps the reference's float semantics: no
 contracted (fused) single-precision operation outside the IEEE division
-expansion (v_div_scale .. v_div_fixup) but those whose product is exact (one
-factor a power of two, which the kernels fold into a sum on purpose), and no
-packed fp32 arithmetic."""
+expansion (v_div_scale .. v_div_fixup) but those whose product is exact for
+every float the kernels see (one factor a power of two of magnitude 1 or more,
+which the kernels fold into a sum on purpose: scaling up is exact short of
+overflow, subnormals included, while halving a subnormal with its last bit set
+rounds), and no packed fp32 arithmetic."""
 import os
 import re
-import struct
 import subprocess
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -39,18 +40,20 @@ def in_division(lines, i):
     return "v_rcp_iflag_f32" in "\n".join(lines[max(0, i - 8):i])
 
 
-INLINE_POW2 = {"0.5", "-0.5", "1.0", "-1.0", "2.0", "-2.0", "4.0", "-4.0"}
+# Inline constants that are powers of two of magnitude 1 or more (0.5 and -0.5 are not: a
+# product by them rounds a subnormal with its last bit set).
+INLINE_POW2 = {"1.0", "-1.0", "2.0", "-2.0", "4.0", "-4.0"}
 
 
 def pow2_bits(text):
-    """Whether a 32-bit constant's float value is plus or minus a power of two."""
+    """Whether a 32-bit constant's float value is plus or minus a power of two of magnitude 1
+    or more (a biased exponent of 127 or more, a zero fraction, finite)."""
     try:
         bits = int(text, 0) & 0xFFFFFFFF
     except ValueError:
         return False
-    value = struct.unpack("<f", struct.pack("<I", bits))[0]
     exponent = (bits >> 23) & 0xFF
-    return value != 0.0 and (bits & 0x7FFFFF) == 0 and 0 < exponent < 255
+    return (bits & 0x7FFFFF) == 0 and 127 <= exponent < 255
 
 
 def exact_product(lines, i):
@@ -101,11 +104,18 @@ def test_no_contraction_in_kernels():
 def test_the_exact_product_rule():
     lines = ["s_mov_b32 s4, 0x49800000", "v_add_f32_e32 v1, v2, v3", "v_fma_f32 v0, v1, s4, 0.5",
              "v_fmac_f32_e32 v0, 2.0, v1", "v_fmac_f32_e32 v0, v3, v1", "v_fma_f32 v0, v1, 0x40400000, v2",
-             "v_fmamk_f32 v0, v1, 0x4f800000, v2", "s_mov_b32 s4, 0x3f000001", "v_fma_f32 v0, v1, s4, v2"]
+             "v_fmamk_f32 v0, v1, 0x4f800000, v2", "s_mov_b32 s4, 0x3f000001", "v_fma_f32 v0, v1, s4, v2",
+             "v_fmac_f32_e32 v0, 0.5, v1", "v_fma_f32 v0, v1, -0.5, v2", "v_fmamk_f32 v0, v1, 0x3e000000, v2",
+             "s_mov_b32 s5, 0x3d800000", "v_fma_f32 v0, v1, s5, v2", "v_fma_f32 v0, -v1, -4.0, v2"]
     assert exact_product(lines, 2) and exact_product(lines, 3) and exact_product(lines, 6)
+    assert exact_product(lines, 14)  # -4.0 on a negated operand
     assert not exact_product(lines, 4)  # two registers
     assert not exact_product(lines, 5)  # 3.0
     assert not exact_product(lines, 8)  # s4 was reset to a non-power of two
+    assert not exact_product(lines, 9)  # 0.5: halving rounds a subnormal
+    assert not exact_product(lines, 10)  # -0.5
+    assert not exact_product(lines, 11)  # 0.125 as a literal
+    assert not exact_product(lines, 13)  # 0.0625 in a scalar register
 
 
 def test_kernels_make_no_calls_and_use_no_scratch():
