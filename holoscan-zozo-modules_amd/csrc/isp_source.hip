@@ -28,6 +28,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "../../include/isp_source.h"
 #include "chart.hpp"
@@ -185,7 +187,7 @@ struct CheckLds {
 __global__ void __launch_bounds__(kSrcThreads) check_kernel(const Chart s, const float* __restrict__ lut, int lut_size,
                                                             const std::uint8_t* __restrict__ rgb, std::size_t pitch,
                                                             const unsigned int* __restrict__ histogram,
-                                                            const float* __restrict__ stats, CheckScratch* scratch,
+                                                            const float* __restrict__ stats, CheckScratch* work,
                                                             isp_frame_check_result* result, int tiles_x, int tiles) {
   __shared__ CheckLds lds;
   const int tid = static_cast<int>(threadIdx.x);
@@ -234,20 +236,20 @@ __global__ void __launch_bounds__(kSrcThreads) check_kernel(const Chart s, const
   if (tid == 0) {
     unsigned long long block = 0ULL;
     for (int w = 0; w < kSrcThreads / 64; ++w) block += lds.wave_err[w];
-    if (block != 0ULL) token ^= fold(atomicAdd(&scratch->err[slot].v, block));
+    if (block != 0ULL) token ^= fold(atomicAdd(&work->err[slot].v, block));
   }
   for (int i = tid; i < kCheckEntries; i += kSrcThreads) {
     unsigned int c = 0U;
 #pragma unroll
     for (int k = 0; k < kCheckCopies; ++k) c += lds.bins[i * kCheckCopies + k];
-    if (c != 0U) token ^= atomicAdd(&scratch->hist[slot][i], c);
+    if (c != 0U) token ^= atomicAdd(&work->hist[slot][i], c);
   }
-  if (!complete_grid(scratch->done, &scratch->top, token, &lds.flag)) return;
+  if (!complete_grid(work->done, &work->top, token, &lds.flag)) return;
   // The last block: the rebuilt histograms against the sent ones (compare_kernel,
   // frame_check.cu:58-70), and the result.
   unsigned int miss = 0U;
   for (int i = tid; i < kCheckEntries; i += kSrcThreads) {
-    const unsigned int rebuilt = take_slots(&scratch->hist[0][i], kCheckEntries);
+    const unsigned int rebuilt = take_slots(&work->hist[0][i], kCheckEntries);
     miss += rebuilt != histogram[i] ? 1U : 0U;
   }
   for (int offset = 32; offset > 0; offset >>= 1) miss += __shfl_down(miss, offset, 64);
@@ -256,12 +258,19 @@ __global__ void __launch_bounds__(kSrcThreads) check_kernel(const Chart s, const
   if (tid == 0) {
     unsigned int total = 0U;
     for (int w = 0; w < kSrcThreads / 64; ++w) total += lds.wave_miss[w];
-    result->squared_error = take_slots(&scratch->err[0].v, sizeof(LineTotal) / sizeof(unsigned long long));
+    result->squared_error = take_slots(&work->err[0].v, sizeof(LineTotal) / sizeof(unsigned long long));
     result->histogram_mismatches = total;
     result->reserved = 0U;
   }
   if (tid < ISP_STATS_SIZE) result->stats[tid] = stats != nullptr ? stats[tid] : 0.0F;
-  reset_grid(scratch->done, &scratch->top);
+  reset_grid(work->done, &work->top);
+}
+
+// Zeroes `words` 32-bit words at `p` (a fill kernel, not a memset node, so that
+// a captured launch replays it correctly).
+__global__ void __launch_bounds__(kSrcThreads) clear_kernel(unsigned int* __restrict__ p, int words) {
+  const int i = static_cast<int>(blockIdx.x) * kSrcThreads + static_cast<int>(threadIdx.x);
+  if (i < words) p[i] = 0U;
 }
 
 int check_blocks_per_cu() {
@@ -272,6 +281,66 @@ int check_blocks_per_cu() {
     return n > 0 ? n : 1;
   }();
   return blocks;
+}
+
+}  // namespace
+
+// The scratch of the allocation-free check (isp_check_frame_device_static): a
+// file-scope device array of one CheckScratch (ISP_FRAME_CHECK_SCRATCH_BYTES,
+// 37,120 bytes), of which every GPU the library runs on holds its own copy
+// until the process ends. Host code never uses the array's name as an address
+// (on the host it is only a stand-in); it uses the device address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[sizeof(CheckScratch)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+CheckScratch* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current. The first lookup also queues a clear of the array on
+// `stream` ahead of the check: the check leaves the array zeroed for the next
+// one, but nothing has zeroed it before the first.
+CheckScratch* static_scratch(hipStream_t stream) {
+  int device = 0;
+  check(hipGetDevice(&device), "hipGetDevice");
+  if (device < 0 || device >= kMaxDevices) throw runtime_failure("IspPipeline (hip): device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device] != nullptr) return g_scratch_ptr[device];
+  void* ptr = nullptr;
+  check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress (frame check scratch)");
+  if (ptr == nullptr) throw runtime_failure("IspPipeline (hip): hipGetSymbolAddress returned no address");
+  constexpr int kWords = static_cast<int>(sizeof(CheckScratch) / sizeof(unsigned int));
+  static_cast<void>(hipGetLastError());
+  clear_kernel<<<dim3((kWords + kSrcThreads - 1) / kSrcThreads), dim3(kSrcThreads), 0, stream>>>(
+      static_cast<unsigned int*>(ptr), kWords);
+  check(hipGetLastError(), "clear_kernel");
+  g_scratch_ptr[device] = static_cast<CheckScratch*>(ptr);
+  return g_scratch_ptr[device];
+}
+
+// The launch of one check with the workspace `work` (validated arguments).
+void launch_check(const isp_synthetic_config* camera, const float* lut, int32_t lut_size, const uint8_t* rgb,
+                  size_t pitch, const uint32_t* histogram, const float* stats, CheckScratch* work,
+                  isp_frame_check_result* result, hipStream_t stream) {
+  const Chart s = make_chart(*camera);
+  if (lut_size < 2) throw invalid_argument("the tone table needs at least 2 entries");
+  const std::size_t packed = 3 * static_cast<std::size_t>(s.width);
+  if (pitch == 0) pitch = packed;
+  if (pitch < packed) throw invalid_argument("pitch is shorter than an RGB8 row");
+  int device = 0;
+  int cus = 1;
+  check(hipGetDevice(&device), "hipGetDevice");
+  check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+  const int tiles_x = (s.width + kSrcTileW - 1) / kSrcTileW;
+  const int tiles = tiles_x * ((s.height + kSrcTileH - 1) / kSrcTileH);
+  const int grid = std::min(tiles, cus * check_blocks_per_cu());
+  static_cast<void>(hipGetLastError());
+  check_kernel<<<dim3(static_cast<unsigned int>(grid)), dim3(kSrcThreads), 0, stream>>>(
+      s, lut, lut_size, rgb, pitch, histogram, stats, work, result, tiles_x, tiles);
+  check(hipGetLastError(), "check_kernel");
 }
 
 }  // namespace
@@ -301,29 +370,37 @@ isp_status isp_synthetic_render_device(const isp_synthetic_config* config, void*
 
 isp_status isp_check_frame_device(const isp_synthetic_config* camera, const float* lut, int32_t lut_size,
                                   const uint8_t* rgb, size_t pitch, const uint32_t* histogram,
-                                  const float* stats, void* scratch, isp_frame_check_result* result,
+                                  const float* stats, void* work, isp_frame_check_result* result,
                                   isp_stream_t stream) {
   return guarded([&] {
-    if (camera == nullptr || lut == nullptr || rgb == nullptr || histogram == nullptr || scratch == nullptr ||
+    if (camera == nullptr || lut == nullptr || rgb == nullptr || histogram == nullptr || work == nullptr ||
         result == nullptr) {
       throw invalid_argument("camera, lut, rgb, histogram, scratch or result is null");
     }
-    const Chart s = make_chart(*camera);
-    if (lut_size < 2) throw invalid_argument("the tone table needs at least 2 entries");
-    const std::size_t packed = 3 * static_cast<std::size_t>(s.width);
-    if (pitch == 0) pitch = packed;
-    if (pitch < packed) throw invalid_argument("pitch is shorter than an RGB8 row");
-    int device = 0;
-    int cus = 1;
-    check(hipGetDevice(&device), "hipGetDevice");
-    check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
-    const int tiles_x = (s.width + kSrcTileW - 1) / kSrcTileW;
-    const int tiles = tiles_x * ((s.height + kSrcTileH - 1) / kSrcTileH);
-    const int grid = std::min(tiles, cus * check_blocks_per_cu());
-    static_cast<void>(hipGetLastError());
-    check_kernel<<<dim3(static_cast<unsigned int>(grid)), dim3(kSrcThreads), 0, static_cast<hipStream_t>(stream)>>>(
-        s, lut, lut_size, rgb, pitch, histogram, stats, static_cast<CheckScratch*>(scratch), result, tiles_x, tiles);
-    check(hipGetLastError(), "check_kernel");
+    launch_check(camera, lut, lut_size, rgb, pitch, histogram, stats, static_cast<CheckScratch*>(work), result,
+                 static_cast<hipStream_t>(stream));
+  });
+}
+
+isp_status isp_check_frame_device_static(const isp_synthetic_config* camera, const float* lut, int32_t lut_size,
+                                         const uint8_t* rgb, size_t pitch, const uint32_t* histogram,
+                                         const float* stats, isp_frame_check_result* result,
+                                         isp_stream_t stream) {
+  return guarded([&] {
+    if (camera == nullptr || lut == nullptr || rgb == nullptr || histogram == nullptr || result == nullptr) {
+      throw invalid_argument("camera, lut, rgb, histogram or result is null");
+    }
+    // The array's 32-bit bin counters take up to a whole frame per slot.
+    const std::uint64_t pixels = static_cast<std::uint64_t>(camera->width > 0 ? camera->width : 0) *
+                                 static_cast<std::uint64_t>(camera->height > 0 ? camera->height : 0);
+    if (pixels > ISP_FRAME_CHECK_STATIC_MAX_PIXELS) {
+      throw invalid_argument("frame of " + std::to_string(pixels) +
+                             " pixels is beyond the static frame check scratch's maximum of " +
+                             std::to_string(static_cast<unsigned long long>(ISP_FRAME_CHECK_STATIC_MAX_PIXELS)) +
+                             " pixels");
+    }
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    launch_check(camera, lut, lut_size, rgb, pitch, histogram, stats, static_scratch(st), result, st);
   });
 }
 

@@ -288,6 +288,8 @@ PROTOTYPES = [
     ("isp_synthetic_render_device", C.c_int, [P(SyntheticConfig), _void, C.c_size_t, _void]),
     ("isp_check_frame_device", C.c_int, [P(SyntheticConfig), _void, C.c_int32, _void, C.c_size_t, _void,
                                          _void, _void, _void, _void]),
+    ("isp_check_frame_device_static", C.c_int, [P(SyntheticConfig), _void, C.c_int32, _void, C.c_size_t,
+                                                _void, _void, _void, _void]),
     ("isp_frame_check_psnr", C.c_double, [P(FrameCheckResult), C.c_int32, C.c_int32]),
     ("isp_color_params_init", None, [P(ColorParams)]),
     ("isp_color_plane_size", C.c_int, [C.c_uint8, C.c_int32, C.c_int32, C.c_int32, P(C.c_size_t), P(C.c_int32)]),

@@ -58,6 +58,22 @@ isp_status isp_check_frame_device(const isp_synthetic_config* camera, const floa
                                   const float* stats, void* scratch, isp_frame_check_result* result,
                                   isp_stream_t stream);
 
+/* No counterpart in the reference: isp_check_frame_device without the scratch
+ * argument, for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It allocates
+ * nothing: its scratch is a static device array of ISP_FRAME_CHECK_SCRATCH_BYTES
+ * that each GPU holds from the library's first use on it until the process
+ * ends, cleared once on that first call and left zeroed by every check. All
+ * calls on one GPU share that array, so calls that overlap on one GPU race:
+ * queue them on one stream, or wait between them. Frames above
+ * ISP_FRAME_CHECK_STATIC_MAX_PIXELS (what the array's 32-bit bin counters hold)
+ * are refused with ISP_INVALID_ARGUMENT. Otherwise as isp_check_frame_device. */
+#define ISP_FRAME_CHECK_STATIC_MAX_PIXELS 4294967295ULL
+isp_status isp_check_frame_device_static(const isp_synthetic_config* camera, const float* lut, int32_t lut_size,
+                                         const uint8_t* rgb, size_t pitch, const uint32_t* histogram,
+                                         const float* stats, isp_frame_check_result* result,
+                                         isp_stream_t stream);
+
 /* The PSNR in dB of a checked frame (check_op.cpp:226-228): 99 when the error is 0. */
 double isp_frame_check_psnr(const isp_frame_check_result* result, int32_t width, int32_t height);
 
