@@ -80,6 +80,11 @@ struct isp_batch {
   bool dirty = false;    // the host table is newer than the device's
   bool sharpen = false;  // a frame sharpens: the batch kernel with the larger LDS
   bool wide = true;      // every frame runs 32-row tiles: the 512-thread kernels
+  // Frames whose tone table starts at -0.0 (FrameDesc::zero_sign): they run the
+  // per-frame kernels, which have the form that keeps every zero's sign, in the
+  // same launch sequence, ahead of the batch kernels; the batch kernels carry
+  // only the common form.
+  std::vector<int> solo;
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -133,8 +138,8 @@ struct isp_batch {
     t.count = n;
     t.queue = schedule.queue;
     sharpen = false;
-    for (const isp_batch_frame& fr : frames) sharpen = sharpen || fr.pipeline->params.c.sharpen > 0.0F;
     wide = true;
+    solo.clear();
     std::vector<double> cost(n);
     for (int i = 0; i < n; ++i) {
       const isp_batch_frame& fr = frames[i];
@@ -146,6 +151,13 @@ struct isp_batch {
       const bool malvar = c.demosaic == ISP_DEMOSAIC_MALVAR;
       const bool sharpen = c.sharpen > 0.0F;
       const int tile_h = schedule.tile_height[fr.raw.format][malvar ? 1 : 0];
+      if (b.f.zero_sign != 0) {
+        // No work items in the batch passes: launch() runs the frame by itself.
+        solo.push_back(i);
+        cost[i] = 0.0;
+        continue;
+      }
+      this->sharpen = this->sharpen || sharpen;
       wide = wide && tile_h == 32;
       b.variant = fr.raw.format | (malvar ? 4 : 0) | (sharpen ? 8 : 0) | (tile_h == 16 ? 16 : 0);
       b.tiles_x = ceil_div(fr.raw.width, kBatchTileW);
@@ -177,6 +189,7 @@ struct isp_batch {
     const int awb_cap = static_cast<int>(std::max(1LL, awb_tiles / (2LL * awb_capacity)));
     for (int i = 0; i < n; ++i) {
       BatchFrame& b = t.frames[i];
+      if (b.tiles == 0) continue;  // a frame launch() runs by itself
       b.tiles_per_item = std::min(b.tiles_per_item, main_cap);
       b.items = ceil_div(b.tiles, b.tiles_per_item);
       if (b.awb_tiles > 0) {
@@ -220,8 +233,14 @@ struct isp_batch {
     const int awb_blocks = h_table->awb_items > 0 ? awb_grid() : 0;
     const int main_blocks = main_grid();
     const auto enqueue = [&](hipStream_t st) {
+      for (int i : solo) {
+        const BatchFrame& b = h_table->frames[i];
+        const isp_pipeline* p = frames[i].pipeline;
+        if (p->params.c.white_balance != ISP_WB_MANUAL) launch_awb(b.f, b.s, p->launch, st);
+        launch_main(b.f, b.s, p->params.c.demosaic == ISP_DEMOSAIC_MALVAR, p->params.c.sharpen > 0.0F, p->launch, st);
+      }
       if (awb_blocks > 0) launch_batch_awb(d_table, d_control, awb_blocks, st);
-      launch_batch_main(d_table, d_control, main_blocks, sharpen, wide, st);
+      if (h_table->main_items > 0) launch_batch_main(d_table, d_control, main_blocks, sharpen, wide, st);
     };
     if (schedule.graph != 0) {
       if (exec == nullptr || graph_grid != main_blocks * 65536 + awb_blocks) {

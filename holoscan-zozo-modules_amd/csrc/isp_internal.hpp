@@ -44,6 +44,17 @@ std::size_t raw_row_bytes(int format, int width);
 void set_error(const std::string& message);
 void clear_error();
 
+// The Bayer layout as the kernels take it: 2 bits per quad position
+// p = (y & 1) * 2 + (x & 1), holding the channel at that position (kCfa,
+// reference cpu_pipeline.cpp:15-20), one code per BayerPattern. The kernels
+// specialise their demosaic on these four codes.
+constexpr int cfa_code(int c0, int c1, int c2, int c3) { return c0 | (c1 << 2) | (c2 << 4) | (c3 << 6); }
+inline constexpr int kCfaCodes[4] = {
+    cfa_code(ISP_CHANNEL_R, ISP_CHANNEL_GR, ISP_CHANNEL_GB, ISP_CHANNEL_B),   // RGGB
+    cfa_code(ISP_CHANNEL_B, ISP_CHANNEL_GB, ISP_CHANNEL_GR, ISP_CHANNEL_R),   // BGGR
+    cfa_code(ISP_CHANNEL_GR, ISP_CHANNEL_R, ISP_CHANNEL_B, ISP_CHANNEL_GB),   // GRBG
+    cfa_code(ISP_CHANNEL_GB, ISP_CHANNEL_B, ISP_CHANNEL_R, ISP_CHANNEL_GR)};  // GBRG
+
 // A camera's exposure carried on the device from one frame to the next (served
 // form): the next frame runs at the previous frame's suggestion. Every pipeline
 // holds one from creation to destruction, so a frame table built earlier (the
@@ -97,6 +108,9 @@ struct FrameDesc {
   // exposure_ns / exposure_gain once valid, and replaced by this frame's
   // suggestion.
   AeState* ae_state;
+  // 1 when the tone table starts at -0.0: the main pass then keeps the sign of
+  // every zero before the table (the kernels' kZeroSign form).
+  int zero_sign;
 };
 
 // Per-pipeline device scratch, zero between frames (each pass's last block clears

@@ -115,6 +115,24 @@ __device__ __forceinline__ int color_of(int channel) {
 // as the reference keeps it): the upper cut as a v_min_f32, the lower as a
 // select, one operation fewer than two selects.
 __device__ __forceinline__ float clamp01(float v) { return v < 0.0F ? 0.0F : fminf(v, 1.0F); }
+// The sign of a zero before the tone table reaches an output only through the
+// table's first entry: tone(+-0) = lut[0] + (+-0) (lut[1] - lut[0]), which is
+// +0 or lut[0] itself for either sign unless lut[0] is -0.0 (a user table;
+// validation reads it as 0). Every other use of these values (products, sums,
+// to_fixed, the output codes) gives the same result for +0 and -0, and no
+// operation of the path turns a zero's sign into a nonzero value. So for a
+// table that does not start at -0.0 (kZeroSign false) a clamp may give either
+// zero for -0: clamp_unit is then med3(v, 0, 1), which the compiler folds into
+// the producing instruction's clamp modifier, and the tone fraction is
+// v_fract_f32. With kZeroSign the reference's forms keep every sign.
+template <bool kZeroSign>
+__device__ __forceinline__ float clamp_unit(float v) {
+  if constexpr (kZeroSign) {
+    return clamp01(v);
+  } else {
+    return __builtin_amdgcn_fmed3f(v, 0.0F, 1.0F);
+  }
+}
 
 // p * a + b for p = 2 or 4, in one instruction. Scaling up by a power of two
 // is exact for every float short of overflow (the values here are at most a few
@@ -731,7 +749,7 @@ __global__ void __launch_bounds__(kThreads) awb_kernel(FrameDesc f, DeviceScratc
 // The two demosaic formulas (isp_math.hpp:161-215) a CFA site needs, from a
 // window w[row][col] whose centre is w[cy][cx]: at an R or B site green and the
 // diagonal colour, at a G site the colour across and the colour down.
-template <bool kMalvar, int kRows, int kCols>
+template <bool kMalvar, bool kZeroSign, int kRows, int kCols>
 __device__ __forceinline__ void demosaic_pair(const float (&w)[kRows][kCols], int cy, int cx, bool rb,
                                               float* first, float* second) {
   const float c = w[cy][cx];
@@ -760,12 +778,14 @@ __device__ __forceinline__ void demosaic_pair(const float (&w)[kRows][kCols], in
     if (rb) {
       const float axial1 = n12 + n32 + n21 + n23;
       const float axial2 = n02 + n42 + n20 + n24;
-      *first = clamp01(exact_fma(2.0F, exact_fma(2.0F, c, axial1), -axial2) * 0.125F);
-      *second = clamp01((exact_fma(2.0F, diagonal1, 6.0F * c) - 1.5F * axial2) * 0.125F);
+      *first = clamp_unit<kZeroSign>(exact_fma(2.0F, exact_fma(2.0F, c, axial1), -axial2) * 0.125F);
+      *second = clamp_unit<kZeroSign>((exact_fma(2.0F, diagonal1, 6.0F * c) - 1.5F * axial2) * 0.125F);
     } else {
       const float horizontal2 = n20 + n24;
-      *first = clamp01((exact_fma(4.0F, n21 + n23, 5.0F * c) - horizontal2 - diagonal1 + 0.5F * vertical2) * 0.125F);
-      *second = clamp01((exact_fma(4.0F, n12 + n32, 5.0F * c) - vertical2 - diagonal1 + 0.5F * horizontal2) * 0.125F);
+      *first = clamp_unit<kZeroSign>(
+          (exact_fma(4.0F, n21 + n23, 5.0F * c) - horizontal2 - diagonal1 + 0.5F * vertical2) * 0.125F);
+      *second = clamp_unit<kZeroSign>(
+          (exact_fma(4.0F, n12 + n32, 5.0F * c) - vertical2 - diagonal1 + 0.5F * horizontal2) * 0.125F);
     }
   }
 }
@@ -807,13 +827,36 @@ __device__ __forceinline__ void assign_rgb(int ch, float c, float first, float s
 // select. The table is read from HBM through the caches: its 16 KB stay in L1
 // and L2, and staging it into each block's LDS measured slower (the copy at
 // every block's start, and the random reads' bank conflicts in LDS).
+// The entries are addressed by a 32-bit byte offset from the table's (uniform)
+// base, which a load takes as is; a 64-bit element index would cost a sign
+// extension and a 64-bit add per lookup.
+template <bool kZeroSign>
 __device__ __forceinline__ float tone(const float* lut, int n1, float x) {
   const float f = x * static_cast<float>(n1);
   const int i = static_cast<int>(f);
-  const float t = f - static_cast<float>(i);
-  const float a = lut[i];
-  const float b = lut[i + 1];
+  // f >= 0 (or -0): f - (float)i is exact, and v_fract_f32 equals it but at -0.
+  float t;
+  if constexpr (kZeroSign) {
+    t = f - static_cast<float>(i);
+  } else {
+    t = __builtin_amdgcn_fractf(f);
+  }
+  const float* e = reinterpret_cast<const float*>(reinterpret_cast<const char*>(lut) + (static_cast<unsigned>(i) << 2));
+  const float a = e[0];
+  const float b = e[1];
   return a + t * (b - a);
+}
+
+// luma_code (isp_math.hpp:290-295) of output codes below 1024:
+// (2126 R + 7152 G + 722 B + 5000) / 10000 in full-rate 24-bit multiplies. The
+// sum n is below 10,235,001 < 2^24, and n / 10000 rounded down is the high
+// word of n * ceil(2^35 / 10000) shifted right by 3, exactly for every such n
+// (the error term n * 0.1632 / 2^35 stays below 1 / 10000).
+__device__ __forceinline__ unsigned int luma_code(unsigned int r, unsigned int g, unsigned int b) {
+  const unsigned int n = __umul24(r, 2126U) + __umul24(g, 7152U) + __umul24(b, 722U) + 5000U;
+  unsigned int hi;
+  __asm__("v_mul_hi_u32_u24 %0, 0x346dc6, %1" : "=v"(hi) : "v"(n));  // 0x346dc6 = ceil(2^35 / 10000)
+  return hi >> 3;
 }
 
 template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
@@ -877,7 +920,8 @@ struct MainCtx {
   const float* grid;
   int n1;
   int bins;
-  int copies;  // interleaved histogram copies: 8 for 256 bins, 2 for 1024
+  int copies;      // interleaved histogram copies: 8 for 256 bins, 2 for 1024
+  int copy_shift;  // log2(copies) + 2: a code's byte offset in its row is (code << copy_shift) | 4 copy
   bool lsc;
   bool want_hist;
   bool want_stats;
@@ -939,6 +983,7 @@ __device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains
   c.want_stats = f.stats != nullptr || (f.ae_state != nullptr && f.ae_state->enabled != 0);
   c.bins = f.bins;
   c.copies = kHistEntries / (4 * c.bins);
+  c.copy_shift = __builtin_ctz(static_cast<unsigned int>(c.copies)) + 2;
   main_set_gains(f, gains, c);
   if (c.want_hist) {
     for (int i = tid; i < kHistEntries; i += kBlock) fl.hist[i] = 0U;
@@ -959,7 +1004,7 @@ constexpr int kLscSlow = 2;
 struct PreRow {
   float b0, b1, i0, i1, g0, g1;
   int cx, y, par, ci;
-  bool count, interior_x;
+  bool count;
 };
 
 // preprocess (isp_math.hpp:151-155) of one row chunk into sample-tile row
@@ -971,6 +1016,7 @@ template <int kFmt, int kLsc, int kPad, int kSW>
 __device__ __forceinline__ void pre_chunk(const FrameDesc& f, const PreRow& r, const int (&raw)[Chunk<kFmt>::kSamples],
                                           const F2* table, float tyr, float* out, unsigned int& saturated) {
   constexpr int kC = Chunk<kFmt>::kSamples;
+  unsigned int sat = 0U;
 #pragma unroll
   for (int h = 0; h < kC / 4; ++h) {
     const int col0 = kPad - kC + r.ci * kC + 4 * h;
@@ -1002,15 +1048,154 @@ __device__ __forceinline__ void pre_chunk(const FrameDesc& f, const PreRow& r, c
       v = v * (odd ? r.g1 : r.g0);
       v = v < 1.0F ? v : 1.0F;
       o.v[m] = v;
-      if (r.count && (r.interior_x || r.cx + k < f.width)) saturated += raw[k] >= f.white ? 1U : 0U;
+      sat += raw[k] >= f.white ? 1U : 0U;
     }
     *reinterpret_cast<F4*>(&out[col0]) = o;
+  }
+  // Only the tile's own samples count. Samples past the frame's right edge were
+  // loaded as 0, below every white level (which exceeds the black level >= 0).
+  saturated += r.count ? sat : 0U;
+}
+
+// The 8-bit histogram code of an RGB16 value (output_codes, isp_math.hpp:282-284):
+// (v * (bins - 1) + 32767) / 65535 for v <= 65535 and bins - 1 <= 1023, in a
+// 24-bit multiply and shifts: for every m < 2^26, m / 65535 rounded down is
+// (m + (m >> 16) + 1) >> 16 (checked for every v at both bin counts).
+__device__ __forceinline__ unsigned int code_of_rgb16(unsigned int v, unsigned int top) {
+  const unsigned int m = __umul24(v, top) + 32767U;
+  return (m + (m >> 16) + 1U) >> 16;
+}
+
+// Stage 2 of a tile with sharpening, for the Bayer layout kCfa (a compile-time
+// channel per quad position): the demosaiced RGB of a kRBW x 2 block of the
+// tile's ring (ring columns rc0.., rows rr0, rr0 + 1) into the three LDS planes
+// (kPlane floats apart). Each site evaluates only its own two formulas and its
+// RGB assignment is a renaming. The window of samples rows rr0 .. rr0 + 1 + 2 kR,
+// columns rc0 .. rc0 + kRBW + 3 comes in 128-bit LDS loads.
+template <bool kMalvar, bool kZeroSign, int kCfa, int kR, int kRBW, int kRW, int kSW, int kPlane>
+__device__ __forceinline__ void ring_block(const float* samp, float* ring, int rc0, int rr0) {
+  constexpr int kWR = 2 + 2 * kR;
+  constexpr int kWC = kRBW + 4;
+  float w[kWR][kWC];
+#pragma unroll
+  for (int r = 0; r < kWR; ++r) {
+#pragma unroll
+    for (int q = 0; q < kWC / 4; ++q) {
+      const F4 a = *reinterpret_cast<const F4*>(&samp[(rr0 + r) * kSW + rc0 + 4 * q]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    F4 planes[3][kRBW / 4];
+#pragma unroll
+    for (int i = 0; i < kRBW; ++i) {
+      // ring pixel parity: x = x0 - 2 + rc0 + i, y = y0 - 1 + rr0 + j (x0, rc0, y0, rr0 even)
+      const int ch = cfa_channel(kCfa, (((1 + j) & 1) << 1) | (i & 1));
+      const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
+      float first, second;
+      demosaic_pair<kMalvar, kZeroSign>(w, kR + j, 2 + i, rb, &first, &second);
+      float rgb[3];
+      assign_rgb(ch, w[kR + j][2 + i], first, second, rgb);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) planes[k][i / 4].v[i % 4] = rgb[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+      for (int q = 0; q < kRBW / 4; ++q) {
+        *reinterpret_cast<F4*>(&ring[k * kPlane + (rr0 + j) * kRW + rc0 + 4 * q]) = planes[k][q];
+      }
+    }
+  }
+}
+
+// ring_block for the Bayer code `cfa` (uniform).
+template <bool kMalvar, bool kZeroSign, int kR, int kRBW, int kRW, int kSW, int kPlane>
+__device__ __forceinline__ void ring_block_any(int cfa, const float* samp, float* ring, int rc0, int rr0) {
+  switch (cfa) {
+    case kCfaCodes[ISP_RGGB]:
+      ring_block<kMalvar, kZeroSign, kCfaCodes[ISP_RGGB], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
+      break;
+    case kCfaCodes[ISP_BGGR]:
+      ring_block<kMalvar, kZeroSign, kCfaCodes[ISP_BGGR], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
+      break;
+    case kCfaCodes[ISP_GRBG]:
+      ring_block<kMalvar, kZeroSign, kCfaCodes[ISP_GRBG], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
+      break;
+    default:
+      ring_block<kMalvar, kZeroSign, kCfaCodes[ISP_GBRG], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
+      break;
+  }
+}
+
+// Demosaic of a thread's 4 x kRows3 output block without sharpening, from its
+// sample window w (centre rows kR .., columns 4 ..), for the Bayer layout kCfa
+// and output rows of parity kYPar + j.
+template <bool kMalvar, bool kZeroSign, int kCfa, int kYPar, int kR, int kRows3, int kWR>
+__device__ __forceinline__ void plain_block(const float (&w)[kWR][12], float (&rgb)[kRows3][4][3]) {
+#pragma unroll
+  for (int j = 0; j < kRows3; ++j) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ch = cfa_channel(kCfa, (((kYPar + j) & 1) << 1) | (i & 1));
+      const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
+      float first, second;
+      demosaic_pair<kMalvar, kZeroSign>(w, kR + j, 4 + i, rb, &first, &second);
+      assign_rgb(ch, w[kR + j][4 + i], first, second, rgb[j][i]);
+    }
+  }
+}
+
+template <bool kMalvar, bool kZeroSign, int kCfa, int kR, int kRows3, int kWR>
+__device__ __forceinline__ void plain_block_par(int ypar, const float (&w)[kWR][12], float (&rgb)[kRows3][4][3]) {
+  if (ypar != 0) {
+    plain_block<kMalvar, kZeroSign, kCfa, 1, kR, kRows3>(w, rgb);
+  } else {
+    plain_block<kMalvar, kZeroSign, kCfa, 0, kR, kRows3>(w, rgb);
+  }
+}
+
+// plain_block for the Bayer code `cfa` and the first row's parity `ypar` (both uniform).
+template <bool kMalvar, bool kZeroSign, int kR, int kRows3, int kWR>
+__device__ __forceinline__ void plain_block_any(int cfa, int ypar, const float (&w)[kWR][12],
+                                                float (&rgb)[kRows3][4][3]) {
+  switch (cfa) {
+    case kCfaCodes[ISP_RGGB]:
+      plain_block_par<kMalvar, kZeroSign, kCfaCodes[ISP_RGGB], kR, kRows3>(ypar, w, rgb);
+      break;
+    case kCfaCodes[ISP_BGGR]:
+      plain_block_par<kMalvar, kZeroSign, kCfaCodes[ISP_BGGR], kR, kRows3>(ypar, w, rgb);
+      break;
+    case kCfaCodes[ISP_GRBG]:
+      plain_block_par<kMalvar, kZeroSign, kCfaCodes[ISP_GRBG], kR, kRows3>(ypar, w, rgb);
+      break;
+    default:
+      plain_block_par<kMalvar, kZeroSign, kCfaCodes[ISP_GBRG], kR, kRows3>(ypar, w, rgb);
+      break;
+  }
+}
+
+// The first output row (within the tile) of thread `tid`'s 4 x kRows3 block.
+// With one row a thread and no sharpening, the rows are dealt so that every
+// wave's four rows have one parity (row 8 (w / 2) + 2 (r % 4) + w % 2 for the
+// r-th group of 16 threads, in wave w = r / 4): the demosaic's site types are
+// then the same across a wave. Otherwise rows follow the threads in order.
+template <bool kSharpen, int kRows3>
+__device__ __forceinline__ int output_row(int tid) {
+  if constexpr (!kSharpen && kRows3 == 1) {
+    const int r = tid >> 4;
+    const int w = r >> 2;
+    return 8 * (w >> 1) + 2 * (r & 3) + (w & 1);
+  } else {
+    return kRows3 * (tid >> 4);
   }
 }
 
 // One kTileH x 64 output tile; statistics sums into acc[0..2], saturated
 // samples into acc[3].
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads, bool kZeroSign = false>
 __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_, MainFrameLds& fl,
                                           MainTileLds<kFmt, kMalvar, kSharpen, kTileH, kBlock>& tl, int tile,
                                           int tiles_x, int fast_rows, int out_aligned, unsigned long long acc[4]) {
@@ -1034,7 +1219,8 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
   const int H = f.height;
   const int bins = c.bins;
   const int copies = c.copies;
-  const int copy = (tid & 63) & (copies - 1);  // this lane's histogram copy
+  const unsigned int copy4 = static_cast<unsigned int>((tid & 63) & (copies - 1)) << 2;  // this lane's copy, in bytes
+  const int copy_shift = c.copy_shift;
   const int tyi = tile / tiles_x;
   const int x0 = (tile - tyi * tiles_x) * 64;
   const int y0 = tyi * kTileH;
@@ -1096,7 +1282,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
     const float g0 = par ? c.qc.gain[2] : c.qc.gain[0];
     const float g1 = par ? c.qc.gain[3] : c.qc.gain[1];
     const bool count = c.want_stats && yv >= y0 && yv < y0 + kTileH && yv < H && ci >= 1 && ci <= 64 / kC;
-    const PreRow row{b0, b1, i0, i1, g0, g1, cx, y, par, ci, count, interior_x};
+    const PreRow row{b0, b1, i0, i1, g0, g1, cx, y, par, ci, count};
     float* out = &samp[sr * kSW];
     if (!c.lsc) {
       pre_chunk<kFmt, kLscNone, kPad, kSW>(f, row, raw, nullptr, 0.0F, out, saturated);
@@ -1124,57 +1310,21 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
   __syncthreads();
 
   // 2. With sharpening: the demosaiced RGB of the tile and a one-pixel ring
-  // (ring column rc = x - x0 + 2, row rr = y - y0 + 1), 8 x 2 pixels per thread.
+  // (ring column rc = x - x0 + 2, row rr = y - y0 + 1), kRBW x 2 pixels per thread.
   if constexpr (kSharpen) {
     if (tid < G::kRingBlocks) {
       constexpr int kRBW = G::kRBW;
       const int by = tid / (kRW / kRBW);
       const int bx = tid - by * (kRW / kRBW);
-      const int rc0 = kRBW * bx;
-      const int rr0 = 2 * by;
-      // window: sample rows rr0 .. rr0 + 1 + 2R, columns rc0 .. rc0 + kRBW + 3
-      constexpr int kWR = 2 + 2 * kR;
-      constexpr int kWC = kRBW + 4;
-      float w[kWR][kWC];
-#pragma unroll
-      for (int r = 0; r < kWR; ++r) {
-#pragma unroll
-        for (int q = 0; q < kWC / 4; ++q) {
-          const F4 a = *reinterpret_cast<const F4*>(&samp[(rr0 + r) * kSW + rc0 + 4 * q]);
-#pragma unroll
-          for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        float rgbs[kRBW][3];
-#pragma unroll
-        for (int i = 0; i < kRBW; ++i) {
-          // ring pixel parity: x = x0 - 2 + rc0 + i, y = y0 - 1 + rr0 + j (x0, rc0, y0, rr0 even)
-          const int ch = cfa_channel(f.cfa, (((1 + j) & 1) << 1) | (i & 1));
-          const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
-          float first, second;
-          demosaic_pair<kMalvar>(w, kR + j, 2 + i, rb, &first, &second);
-          assign_rgb(ch, w[kR + j][2 + i], first, second, rgbs[i]);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-#pragma unroll
-          for (int q = 0; q < kRBW / 4; ++q) {
-            F4 o;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) o.v[i] = rgbs[4 * q + i][k];
-            *reinterpret_cast<F4*>(&tl.ring[k][(rr0 + j) * kRW + rc0 + 4 * q]) = o;
-          }
-        }
-      }
+      ring_block_any<kMalvar, kZeroSign, kR, kRBW, kRW, kSW, G::kRH * kRW>(__builtin_amdgcn_readfirstlane(f.cfa), samp,
+                                                                 &tl.ring[0][0], kRBW * bx, 2 * by);
     }
     __syncthreads();
   }
 
   // 3. A 4 x kRows3 block of output pixels per thread.
   const int px = 4 * (tid & 15);
-  const int py = kRows3 * (tid >> 4);
+  const int py = output_row<kSharpen, kRows3>(tid);
   const int xo = x0 + px;
   float rgb[kRows3][4][3];
   if constexpr (kSharpen) {
@@ -1201,7 +1351,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
           const float axial = m[j][i + 2] + m[j + 2][i + 2] + m[j + 1][i + 1] + m[j + 1][i + 3];
           const float diagonal = m[j][i + 1] + m[j][i + 3] + m[j + 2][i + 1] + m[j + 2][i + 3];
           const float blur = exact_fma(2.0F, exact_fma(2.0F, cc, axial), diagonal) * 0.0625F;
-          rgb[j][i][k] = clamp01(cc + f.sharpen * (cc - blur));
+          rgb[j][i][k] = clamp_unit<kZeroSign>(cc + f.sharpen * (cc - blur));
         }
       }
     }
@@ -1218,18 +1368,9 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
         for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
       }
     }
-#pragma unroll
-    for (int j = 0; j < kRows3; ++j) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        // y0 is even; py is even for 32-row tiles and may be odd for 16-row tiles.
-        const int ch = cfa_channel(f.cfa, (((py + j) & 1) << 1) | (i & 1));
-        const bool rb = ch == ISP_CHANNEL_R || ch == ISP_CHANNEL_B;
-        float first, second;
-        demosaic_pair<kMalvar>(w, kR + j, 4 + i, rb, &first, &second);
-        assign_rgb(ch, w[kR + j][4 + i], first, second, rgb[j][i]);
-      }
-    }
+    // y0 is even, and the row parity of py is the same across a wave (output_row).
+    plain_block_any<kMalvar, kZeroSign, kR, kRows3>(__builtin_amdgcn_readfirstlane(f.cfa),
+                                         __builtin_amdgcn_readfirstlane(py & 1), w, rgb);
   }
   const bool full_x = xo + 4 <= W;
 #pragma unroll
@@ -1239,13 +1380,13 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
     // correct_color, to_fixed sums, tone, store, output codes (isp_math.hpp:229-305)
     float lin[4][3];
     float t[4][3];
-    int q[4][3];
+    unsigned int q[4][3];
     unsigned int sums[3] = {0U, 0U, 0U};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        lin[i][k] = clamp01(f.ccm[3 * k] * rgb[j][i][0] + f.ccm[3 * k + 1] * rgb[j][i][1] +
+        lin[i][k] = clamp_unit<kZeroSign>(f.ccm[3 * k] * rgb[j][i][0] + f.ccm[3 * k + 1] * rgb[j][i][1] +
                             f.ccm[3 * k + 2] * rgb[j][i][2]);
       }
       if (c.want_stats && (full_x || xo + i < W)) {
@@ -1256,7 +1397,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) t[i][k] = tone(f.lut, c.n1, lin[i][k]);
+      for (int k = 0; k < 3; ++k) t[i][k] = tone<kZeroSign>(f.lut, c.n1, lin[i][k]);
     }
     if (c.want_stats) {
 #pragma unroll
@@ -1272,7 +1413,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
             const unsigned int v = static_cast<unsigned char>(t[i][k] * 255.0F + 0.5F);
-            q[i][k] = static_cast<int>(v);
+            q[i][k] = v;
             byte[3 * i + k] = v;
           }
         }
@@ -1301,7 +1442,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
             const unsigned int v = static_cast<unsigned char>(t[i][k] * 255.0F + 0.5F);
-            q[i][k] = static_cast<int>(v);
+            q[i][k] = v;
             word[i] |= v << (8 * k);
           }
         }
@@ -1328,7 +1469,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
           for (int k = 0; k < 3; ++k) {
             const unsigned int v = static_cast<std::uint16_t>(t[i][k] * 65535.0F + 0.5F);
             half[3 * i + k] = v;
-            q[i][k] = static_cast<int>((v * static_cast<unsigned int>(bins - 1) + 32767U) / 65535U);
+            q[i][k] = code_of_rgb16(v, static_cast<unsigned int>(bins - 1));
           }
         }
         if (whole) {
@@ -1357,7 +1498,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            q[i][k] = static_cast<int>(clamp01(t[i][k]) * static_cast<float>(bins - 1) + 0.5F);
+            q[i][k] = static_cast<unsigned int>(clamp01(t[i][k]) * static_cast<float>(bins - 1) + 0.5F);
           }
         }
         if (whole) {
@@ -1380,14 +1521,19 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
       }
     }
     if (c.want_hist) {
+      // Row r of the histograms starts at entry r * kHistEntries / 4 (bins x
+      // copies is kHistEntries / 4 for either bin count): a code's entry is a
+      // shift and an or from the row's base, which the LDS instruction's offset holds.
+      unsigned char* h = reinterpret_cast<unsigned char*>(fl.hist);
+      constexpr int kRowBytes = kHistEntries;  // kHistEntries / 4 entries of 4 bytes
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         if (!full_x && xo + i >= W) continue;
-        const int luma = (2126 * q[i][0] + 7152 * q[i][1] + 722 * q[i][2] + 5000) / 10000;
-        atomicAdd(&fl.hist[luma * copies + copy], 1U);
-        atomicAdd(&fl.hist[(bins + q[i][0]) * copies + copy], 1U);
-        atomicAdd(&fl.hist[(2 * bins + q[i][1]) * copies + copy], 1U);
-        atomicAdd(&fl.hist[(3 * bins + q[i][2]) * copies + copy], 1U);
+        const unsigned int luma = luma_code(q[i][0], q[i][1], q[i][2]);
+        atomicAdd(reinterpret_cast<unsigned int*>(h + ((luma << copy_shift) | copy4)), 1U);
+        atomicAdd(reinterpret_cast<unsigned int*>(h + kRowBytes + ((q[i][0] << copy_shift) | copy4)), 1U);
+        atomicAdd(reinterpret_cast<unsigned int*>(h + 2 * kRowBytes + ((q[i][1] << copy_shift) | copy4)), 1U);
+        atomicAdd(reinterpret_cast<unsigned int*>(h + 3 * kRowBytes + ((q[i][2] << copy_shift) | copy4)), 1U);
       }
     }
   }
@@ -1477,7 +1623,7 @@ constexpr int main_waves_per_eu() {
 // kBlock threads per block: 512-thread blocks (one output row per thread) put
 // twice the waves on a CU for the same LDS, with registers held to what
 // main_waves_per_eu() leaves.
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock>
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock, bool kZeroSign>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(kBlock >= 512 ? main_waves_per_eu<kFmt, kSharpen>() : 1)))
 main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows, int out_aligned) {
@@ -1502,7 +1648,8 @@ main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows
   }
   unsigned long long acc[4] = {0, 0, 0, 0};
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock>(f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
+    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign>(f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned,
+                                                                  acc);
   }
   const unsigned int token = main_flush<kBlock>(&s->main, fl, c, acc);
   if (!complete_grid(s->main.done, &s->main.top, token, &fl.flag)) return;
@@ -1876,7 +2023,8 @@ constexpr int kMainBlock = 512;
 template <int kFmt, bool kMalvar, bool kSharpen>
 void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream) {
   constexpr int kBlock = kMainBlock<kSharpen>;
-  auto kernel = main_kernel<kFmt, kMalvar, kSharpen, kMainTileH, kBlock>;
+  auto kernel = f.zero_sign != 0 ? main_kernel<kFmt, kMalvar, kSharpen, kMainTileH, kBlock, true>
+                                  : main_kernel<kFmt, kMalvar, kSharpen, kMainTileH, kBlock, false>;
   const int tiles_x = (f.width + 63) / 64;
   const int tiles_y = (f.height + kMainTileH - 1) / kMainTileH;
   const int tiles = tiles_x * tiles_y;

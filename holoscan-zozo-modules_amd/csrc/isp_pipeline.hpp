@@ -31,6 +31,14 @@ inline constexpr int kCfa[4][4] = {
     {ISP_CHANNEL_GB, ISP_CHANNEL_B, ISP_CHANNEL_R, ISP_CHANNEL_GR},
 };
 
+constexpr bool cfa_codes_match_table() {
+  for (int p = 0; p < 4; ++p) {
+    if (kCfaCodes[p] != cfa_code(kCfa[p][0], kCfa[p][1], kCfa[p][2], kCfa[p][3])) return false;
+  }
+  return true;
+}
+static_assert(cfa_codes_match_table(), "the kernels' Bayer codes follow the CFA table");
+
 // Runs `body`, mapping its exceptions to status codes and the thread's last error.
 template <typename F>
 isp_status guarded(F&& body) {
@@ -139,8 +147,7 @@ struct isp_pipeline {
     f.raw = static_cast<const std::uint8_t*>(raw.data);
     f.raw_pitch = static_cast<long long>(raw.pitch);
     f.raw_format = raw.format;
-    f.cfa = 0;
-    for (int k = 0; k < 4; ++k) f.cfa |= isp_amd::kCfa[raw.pattern][k] << (2 * k);
+    f.cfa = isp_amd::kCfaCodes[raw.pattern];
     const float white = p.white_level > 0.0F ? p.white_level
                                              : static_cast<float>((1U << static_cast<unsigned>(raw.bit_depth)) - 1U);
     for (int k = 0; k < 4; ++k) {
@@ -175,6 +182,7 @@ struct isp_pipeline {
     f.percentile = p.white_patch_percentile;
     f.smoothing = p.awb_smoothing;
     f.ae_state = d_ae;
+    f.zero_sign = lut.front() == 0.0F && std::signbit(lut.front()) ? 1 : 0;
     return f;
   }
 };

@@ -271,6 +271,35 @@ def test_feedback_switched_after_the_frames_are_bound(torch_gpu):
             assert pipes[0].exposure_state() == singles[0].exposure_state()
 
 
+def zero_sign_spec(seed):
+    """A stream whose tone table starts at -0.0, with the signed zeros it can carry: many samples
+    below black, a colour matrix row of negative weights, RGB32F output."""
+    sc = cases.baseline_scene(96, 70, _abi.RAW_U16, 12, _abi.GRBG, 600.0, seed)
+    lut = np.linspace(0.0, 1.0, 257, dtype=np.float32)
+    lut[0] = np.float32(-0.0)
+    p = isp.IspParams(black_level=600.0, wb_gains=(1.0, 1.0, 1.0), demosaic=_abi.MALVAR, sharpen=1.5,
+                      ccm=(1.0, 0.0, 0.0, 0.0, 1.0, 0.0, -0.25, -0.5, -0.25), tone_curve=_abi.TONE_LUT,
+                      lut=[float(v) for v in lut], white_balance=_abi.WB_GRAY_WORLD)
+    return sc, p, _abi.OUT_RGB32F
+
+
+def test_zero_sign_frames_in_a_batch(torch_gpu):
+    """Streams whose tone table starts at -0.0 run the per-frame kernels inside the batch's launch
+    sequence: the batch equals per-frame processing and the oracle, bit for bit (signed zeros of
+    the RGB32F output included), on eager and captured launches."""
+    specs = mixed_specs()[:3] + [zero_sign_spec(41), zero_sign_spec(42)]
+    oracle = run_oracle(specs, ticks=2)
+    single = run_single(specs, ticks=2)
+    for graph in (0, 1):
+        batched = run_batch(specs, make_schedule(graph=graph), ticks=2)
+        for t in range(2):
+            for i in range(len(specs)):
+                compare(f"graph {graph} tick {t} stream {i} vs oracle", batched[t][i], oracle[t][i])
+                compare(f"graph {graph} tick {t} stream {i} vs per-frame", batched[t][i], single[t][i])
+    negative = [np.count_nonzero(r.image.view(np.uint32) == 0x80000000) for r in oracle[0][3:]]
+    assert min(negative) > 0  # the outputs hold -0.0
+
+
 def test_autotuner_cache_and_state(torch_gpu, tmp_path):
     """The tuner returns a valid schedule, stores it, reuses it without timing, and leaves the
     streams' white balance state as it found it."""
