@@ -93,6 +93,14 @@ struct alignas(8) F2 {
   float v[2];
 };
 
+// 16 bytes of LDS read as a native 16-byte-aligned vector, which becomes one
+// ds_read_b128 where all four elements are used. gfx950's LDS serves a wave's
+// 128-bit reads at a 16-byte lane stride without bank conflicts; the same bytes
+// as ds_read2_b64 (what the compiler makes of a struct copy it can only prove
+// 8-byte aligned) cost several conflict cycles.
+typedef float V4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ V4 lds_v4(const float* p) { return *reinterpret_cast<const V4*>(p); }
+
 __device__ __forceinline__ int mirror_any(int v, int n) {
   const int period = 2 * (n - 1);
   v %= period;
@@ -847,6 +855,41 @@ __device__ __forceinline__ float tone(const float* lut, int n1, float x) {
   return a + t * (b - a);
 }
 
+// tone of a thread's 4 x 3 values, its twelve table reads issued before any is
+// used (each read waits on the caches; issued one pair at a time they would
+// queue behind each other).
+template <bool kZeroSign>
+__device__ __forceinline__ void tone12(const float* lut, int n1, const float (&x)[4][3], float (&out)[4][3]) {
+  // In two groups of six reads: all twelve in flight at once would hold 24
+  // registers of results beside the pass's other live values.
+#pragma unroll
+  for (int g = 0; g < 12; g += 6) {
+    float frac[6];
+    unsigned int off[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const float f = x[(g + m) / 3][(g + m) % 3] * static_cast<float>(n1);
+      const int i = static_cast<int>(f);
+      if constexpr (kZeroSign) {
+        frac[m] = f - static_cast<float>(i);
+      } else {
+        frac[m] = __builtin_amdgcn_fractf(f);
+      }
+      off[m] = static_cast<unsigned int>(i) << 2;
+    }
+    float a[6];
+    float b[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const float* e = reinterpret_cast<const float*>(reinterpret_cast<const char*>(lut) + off[m]);
+      a[m] = e[0];
+      b[m] = e[1];
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) out[(g + m) / 3][(g + m) % 3] = a[m] + frac[m] * (b[m] - a[m]);
+  }
+}
+
 // luma_code (isp_math.hpp:290-295) of output codes below 1024:
 // (2126 R + 7152 G + 722 B + 5000) / 10000 in full-rate 24-bit multiplies. The
 // sum n is below 10,235,001 < 2^24, and n / 10000 rounded down is the high
@@ -1072,7 +1115,7 @@ __device__ __forceinline__ unsigned int code_of_rgb16(unsigned int v, unsigned i
 // (kPlane floats apart). Each site evaluates only its own two formulas and its
 // RGB assignment is a renaming. The window of samples rows rr0 .. rr0 + 1 + 2 kR,
 // columns rc0 .. rc0 + kRBW + 3 comes in 128-bit LDS loads.
-template <bool kMalvar, bool kZeroSign, int kCfa, int kR, int kRBW, int kRW, int kSW, int kPlane>
+template <bool kMalvar, bool kZeroSign, bool kWhole, int kCfa, int kR, int kRBW, int kRW, int kSW, int kPlane>
 __device__ __forceinline__ void ring_block(const float* samp, float* ring, int rc0, int rr0) {
   constexpr int kWR = 2 + 2 * kR;
   constexpr int kWC = kRBW + 4;
@@ -1081,9 +1124,16 @@ __device__ __forceinline__ void ring_block(const float* samp, float* ring, int r
   for (int r = 0; r < kWR; ++r) {
 #pragma unroll
     for (int q = 0; q < kWC / 4; ++q) {
-      const F4 a = *reinterpret_cast<const F4*>(&samp[(rr0 + r) * kSW + rc0 + 4 * q]);
+      const float* p = &samp[(rr0 + r) * kSW + rc0 + 4 * q];
+      if constexpr (kWhole) {
+        const V4 a = lds_v4(p);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
+        for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a[k];
+      } else {
+        const F4 a = *reinterpret_cast<const F4*>(p);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
+      }
     }
   }
 #pragma unroll
@@ -1112,20 +1162,20 @@ __device__ __forceinline__ void ring_block(const float* samp, float* ring, int r
 }
 
 // ring_block for the Bayer code `cfa` (uniform).
-template <bool kMalvar, bool kZeroSign, int kR, int kRBW, int kRW, int kSW, int kPlane>
+template <bool kMalvar, bool kZeroSign, bool kWhole, int kR, int kRBW, int kRW, int kSW, int kPlane>
 __device__ __forceinline__ void ring_block_any(int cfa, const float* samp, float* ring, int rc0, int rr0) {
   switch (cfa) {
     case kCfaCodes[ISP_RGGB]:
-      ring_block<kMalvar, kZeroSign, kCfaCodes[ISP_RGGB], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
+      ring_block<kMalvar, kZeroSign, kWhole, kCfaCodes[ISP_RGGB], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
       break;
     case kCfaCodes[ISP_BGGR]:
-      ring_block<kMalvar, kZeroSign, kCfaCodes[ISP_BGGR], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
+      ring_block<kMalvar, kZeroSign, kWhole, kCfaCodes[ISP_BGGR], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
       break;
     case kCfaCodes[ISP_GRBG]:
-      ring_block<kMalvar, kZeroSign, kCfaCodes[ISP_GRBG], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
+      ring_block<kMalvar, kZeroSign, kWhole, kCfaCodes[ISP_GRBG], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
       break;
     default:
-      ring_block<kMalvar, kZeroSign, kCfaCodes[ISP_GBRG], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
+      ring_block<kMalvar, kZeroSign, kWhole, kCfaCodes[ISP_GBRG], kR, kRBW, kRW, kSW, kPlane>(samp, ring, rc0, rr0);
       break;
   }
 }
@@ -1193,9 +1243,24 @@ __device__ __forceinline__ int output_row(int tid) {
   }
 }
 
+// Waves per SIMD the main pass's registers are held to: four (two 512-thread
+// blocks a CU); six (three blocks) without sharpening for u8 and u16 samples,
+// whose tile LDS (52 KB) lets three blocks in and whose bodies fit 80 VGPRs
+// without spilling (the packed formats' unpacking would spill). Three blocks
+// made the 1080p plain frame 8 % faster and the 4K one 7-9 %.
+template <int kFmt, bool kSharpen>
+constexpr int main_waves_per_eu() {
+  return !kSharpen && (kFmt == ISP_RAW_U8 || kFmt == ISP_RAW_U16) ? 6 : 4;
+}
+
 // One kTileH x 64 output tile; statistics sums into acc[0..2], saturated
 // samples into acc[3].
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads, bool kZeroSign = false>
+// kSpare: a per-frame kernel at four waves a SIMD, with registers to spare for
+// the tone table's twelve reads in flight and whole 128-bit window loads (the
+// batch kernels, with every tile body in one function, and the six-wave kernels
+// have none).
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads, bool kZeroSign = false,
+          bool kSpare = false>
 __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_, MainFrameLds& fl,
                                           MainTileLds<kFmt, kMalvar, kSharpen, kTileH, kBlock>& tl, int tile,
                                           int tiles_x, int fast_rows, int out_aligned, unsigned long long acc[4]) {
@@ -1316,7 +1381,7 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
       constexpr int kRBW = G::kRBW;
       const int by = tid / (kRW / kRBW);
       const int bx = tid - by * (kRW / kRBW);
-      ring_block_any<kMalvar, kZeroSign, kR, kRBW, kRW, kSW, G::kRH * kRW>(__builtin_amdgcn_readfirstlane(f.cfa), samp,
+      ring_block_any<kMalvar, kZeroSign, kSpare, kR, kRBW, kRW, kSW, G::kRH * kRW>(__builtin_amdgcn_readfirstlane(f.cfa), samp,
                                                                  &tl.ring[0][0], kRBW * bx, 2 * by);
     }
     __syncthreads();
@@ -1336,9 +1401,16 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
       for (int r = 0; r < kRows3 + 2; ++r) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-          const F4 a = *reinterpret_cast<const F4*>(&tl.ring[k][(py + r) * kRW + px + 4 * q]);
+          const float* p = &tl.ring[k][(py + r) * kRW + px + 4 * q];
+          if constexpr (kSpare) {
+            const V4 a = lds_v4(p);
 #pragma unroll
-          for (int e = 0; e < 4; ++e) m[r][4 * q + e] = a.v[e];
+            for (int e = 0; e < 4; ++e) m[r][4 * q + e] = a[e];
+          } else {
+            const F4 a = *reinterpret_cast<const F4*>(p);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) m[r][4 * q + e] = a.v[e];
+          }
         }
       }
 #pragma unroll
@@ -1394,10 +1466,11 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
         for (int k = 0; k < 3; ++k) sums[k] += to_fixed(lin[i][k]);
       }
     }
+    if constexpr (kSpare) {
+      tone12<kZeroSign>(f.lut, c.n1, lin, t);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) t[i][k] = tone<kZeroSign>(f.lut, c.n1, lin[i][k]);
+      for (int m = 0; m < 12; ++m) t[m / 3][m % 3] = tone<kZeroSign>(f.lut, c.n1, lin[m / 3][m % 3]);
     }
     if (c.want_stats) {
 #pragma unroll
@@ -1610,15 +1683,6 @@ __device__ __forceinline__ void main_finish(const FrameDesc& f, MainScratch* m, 
   }
 }
 
-// Waves per SIMD the main pass's registers are held to: four (two 512-thread
-// blocks a CU); six (three blocks) without sharpening for u8 and u16 samples,
-// whose tile LDS (52 KB) lets three blocks in and whose bodies fit 80 VGPRs
-// without spilling (the packed formats' unpacking would spill). Three blocks
-// made the 1080p plain frame 8 % faster and the 4K one 7-9 %.
-template <int kFmt, bool kSharpen>
-constexpr int main_waves_per_eu() {
-  return !kSharpen && (kFmt == ISP_RAW_U8 || kFmt == ISP_RAW_U16) ? 6 : 4;
-}
 
 // kBlock threads per block: 512-thread blocks (one output row per thread) put
 // twice the waves on a CU for the same LDS, with registers held to what
@@ -1648,8 +1712,8 @@ main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows
   }
   unsigned long long acc[4] = {0, 0, 0, 0};
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign>(f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned,
-                                                                  acc);
+    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign, main_waves_per_eu<kFmt, kSharpen>() <= 4>(
+        f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
   }
   const unsigned int token = main_flush<kBlock>(&s->main, fl, c, acc);
   if (!complete_grid(s->main.done, &s->main.top, token, &fl.flag)) return;
