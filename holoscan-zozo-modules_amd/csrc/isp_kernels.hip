@@ -1243,6 +1243,9 @@ __device__ __forceinline__ int output_row(int tid) {
   }
 }
 
+// XCDs of the chip; blocks are dispatched to them in turn.
+constexpr int kXcds = 8;
+
 // Waves per SIMD the main pass's registers are held to: four (two 512-thread
 // blocks a CU); six (three blocks) without sharpening for u8 and u16 samples,
 // whose tile LDS (52 KB) lets three blocks in and whose bodies fit 80 VGPRs
@@ -1435,9 +1438,16 @@ __device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_
     for (int r = 0; r < kWR; ++r) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const F4 a = *reinterpret_cast<const F4*>(&samp[(py + kH - kR + r) * kSW + px + 4 * q]);
+        const float* p = &samp[(py + kH - kR + r) * kSW + px + 4 * q];
+        if constexpr (kSpare) {
+          const V4 a = lds_v4(p);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
+          for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a[k];
+        } else {
+          const F4 a = *reinterpret_cast<const F4*>(p);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) w[r][4 * q + k] = a.v[k];
+        }
       }
     }
     // y0 is even, and the row parity of py is the same across a wave (output_row).
@@ -1711,7 +1721,16 @@ main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows
     main_set_gains(f, g, c);
   }
   unsigned long long acc[4] = {0, 0, 0, 0};
-  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+  // Tiles by XCD: blocks are dispatched to the eight XCDs in turn (block b to
+  // XCD b % 8), and each XCD's blocks sweep one contiguous band of tiles, so
+  // that neighbouring tiles, whose halos overlap, are read through one L2.
+  const int g = static_cast<int>(gridDim.x);
+  const int parts = g < kXcds ? g : kXcds;
+  const int xcd = static_cast<int>(blockIdx.x) % parts;
+  const int members = (g - xcd + parts - 1) / parts;
+  const int band_end = static_cast<int>(static_cast<long long>(tiles) * (xcd + 1) / parts);
+  const int band_begin = static_cast<int>(static_cast<long long>(tiles) * xcd / parts);
+  for (int tile = band_begin + static_cast<int>(blockIdx.x) / parts; tile < band_end; tile += members) {
     main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign, main_waves_per_eu<kFmt, kSharpen>() <= 4>(
         f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
   }
@@ -1802,7 +1821,6 @@ __device__ __forceinline__ int batch_locate(const int* start, const int* order, 
 //              dispatched to the 8 XCDs in turn, block b to XCD b % 8), each
 //              range taken in turn by that XCD's blocks, so neighbouring tiles,
 //              whose halos overlap, meet in the same L2.
-constexpr int kXcds = 8;
 
 struct BatchQueue {
   unsigned int* next;
