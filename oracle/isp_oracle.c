@@ -647,6 +647,65 @@ static void run_bands(band_job* jobs, int threads, int pass, int rows) {
   for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
 }
 
+/* The white balance statistics of a frame (cpu_pipeline.cpp:101-119): the
+ * white-balance pass over row bands of quads, the bands' sums and histograms
+ * added into `a`. */
+static int measure_white_balance(band_job* jobs, int threads, const frame_params* p, awb_accum* a) {
+  awb_accum* parts = (awb_accum*)calloc((size_t)threads, sizeof(awb_accum));
+  if (parts == NULL) return 1;
+  for (int t = 0; t < threads; ++t) jobs[t].awb = &parts[t];
+  run_bands(jobs, threads, 0, p->height / 2);
+  *a = parts[0];
+  for (int t = 1; t < threads; ++t) {
+    for (int k = 0; k < 3; ++k) a->sum[k] += parts[t].sum[k];
+    a->quads += parts[t].quads;
+    for (int k = 0; k < 3; ++k)
+      for (int b = 0; b < AWB_BINS; ++b) a->hist[k][b] += parts[t].hist[k][b];
+  }
+  for (int t = 0; t < threads; ++t) jobs[t].awb = NULL;
+  free(parts);
+  return 0;
+}
+
+struct oracle_awb_stats {
+  awb_accum a;
+};
+
+oracle_awb_stats* oracle_awb_measure(const oracle_pipeline* o, const isp_raw_view* raw, int threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  const isp_rgb_view none = {NULL, 0, 0};
+  const frame_params fp = make_frame_params(o, raw, &none);
+  band_job* jobs = (band_job*)calloc((size_t)threads, sizeof(band_job));
+  oracle_awb_stats* s = (oracle_awb_stats*)malloc(sizeof(oracle_awb_stats));
+  if (jobs == NULL || s == NULL) {
+    free(jobs);
+    free(s);
+    return NULL;
+  }
+  for (int t = 0; t < threads; ++t) {
+    jobs[t].p = &fp;
+    jobs[t].o = o;
+  }
+  const int rc = measure_white_balance(jobs, threads, &fp, &s->a);
+  free(jobs);
+  if (rc != 0) {
+    free(s);
+    return NULL;
+  }
+  return s;
+}
+
+/* finish_white_balance with the pipeline's settings (cpu_pipeline.cpp:120-122); nothing under
+ * manual white balance, as in oracle_process. */
+void oracle_awb_apply(oracle_pipeline* o, const oracle_awb_stats* s) {
+  if (o->params.white_balance == ISP_WB_MANUAL) return; /* oracle_process keeps the state */
+  finish_white_balance(&s->a, o->params.white_balance, o->params.white_patch_percentile,
+                       o->params.awb_smoothing, o->gains, &o->valid);
+}
+
+void oracle_awb_free(oracle_awb_stats* s) { free(s); }
+
 /* cpu_process (cpu_pipeline.cpp:97-191). */
 int oracle_process(oracle_pipeline* o, const isp_raw_view* raw, const isp_rgb_view* out,
                    const isp_stats_out* stats, const isp_frame_exposure* exposure, int threads) {
@@ -671,20 +730,16 @@ int oracle_process(oracle_pipeline* o, const isp_raw_view* raw, const isp_rgb_vi
 
   /* White balance: this frame's estimate, from its own statistics. */
   if (params->white_balance != ISP_WB_MANUAL) {
-    awb_accum* parts = (awb_accum*)calloc((size_t)threads, sizeof(awb_accum));
-    for (int t = 0; t < threads; ++t) jobs[t].awb = &parts[t];
-    run_bands(jobs, threads, 0, p->height / 2);
-    awb_accum* a = &parts[0];
-    for (int t = 1; t < threads; ++t) {
-      for (int k = 0; k < 3; ++k) a->sum[k] += parts[t].sum[k];
-      a->quads += parts[t].quads;
-      for (int k = 0; k < 3; ++k)
-        for (int b = 0; b < AWB_BINS; ++b) a->hist[k][b] += parts[t].hist[k][b];
+    awb_accum* a = (awb_accum*)malloc(sizeof(awb_accum));
+    if (a == NULL || measure_white_balance(jobs, threads, p, a) != 0) {
+      free(a);
+      free(jobs);
+      return 1;
     }
     finish_white_balance(a, params->white_balance, params->white_patch_percentile,
                          params->awb_smoothing, o->gains, &o->valid);
     for (int k = 0; k < 3; ++k) gains[k] = o->gains[k];
-    free(parts);
+    free(a);
   }
 
   /* Black level, lens shading, white balance. */

@@ -38,6 +38,16 @@ int oracle_process(oracle_pipeline* pipeline, const isp_raw_view* raw, const isp
 /* The white balance state after the last frame: gains[3], valid. */
 void oracle_awb_state(const oracle_pipeline* pipeline, float gains[3], int* valid);
 
+/* The white balance estimate split in two, for replaying a long sequence of
+ * frames cheaply: the statistics of one frame (the white-balance pass of
+ * oracle_process, cpu_pipeline.cpp:101-119), and their effect on the state
+ * (finish_white_balance, cpu_pipeline.cpp:120-122). Measuring frame n and
+ * applying it leaves the state oracle_process(frame n) leaves. */
+typedef struct oracle_awb_stats oracle_awb_stats;
+oracle_awb_stats* oracle_awb_measure(const oracle_pipeline* pipeline, const isp_raw_view* raw, int threads);
+void oracle_awb_apply(oracle_pipeline* pipeline, const oracle_awb_stats* stats);
+void oracle_awb_free(oracle_awb_stats* stats);
+
 /* Pieces, for unit tests against closed forms. */
 int oracle_tone_lut(const isp_params* params, float* out, int capacity);
 int oracle_mirror(int v, int n);

@@ -24,6 +24,9 @@ ORACLE_PROTOTYPES = [
     ("oracle_process", C.c_int, [_v, P(_abi.RawView), P(_abi.RgbView), P(_abi.StatsOut),
                                  P(_abi.FrameExposure), C.c_int]),
     ("oracle_awb_state", None, [_v, P(C.c_float), P(C.c_int)]),
+    ("oracle_awb_measure", _v, [_v, P(_abi.RawView), C.c_int]),
+    ("oracle_awb_apply", None, [_v, _v]),
+    ("oracle_awb_free", None, [_v]),
     ("oracle_tone_lut", C.c_int, [P(_abi.Params), P(C.c_float), C.c_int]),
     ("oracle_mirror", C.c_int, [C.c_int, C.c_int]),
     ("oracle_load_raw", C.c_int, [P(_abi.RawView), C.c_int, C.c_int]),
@@ -143,6 +146,28 @@ class OraclePipeline:
         if rc != 0:
             raise RuntimeError("oracle_process failed")
         return Result(image, hist, stats)
+
+    def awb_state(self):
+        g = (C.c_float * 3)()
+        v = C.c_int()
+        oracle().oracle_awb_state(self._h, g, C.byref(v))
+        return list(g), v.value
+
+    def replay_white_balance(self, frames, sequence, threads=1):
+        """The white balance state after running frames[i] for each i in `sequence`, in order,
+        without their main passes: each distinct frame's statistics are measured once."""
+        stats = {}
+        try:
+            for i in set(sequence):
+                s = oracle().oracle_awb_measure(self._h, C.byref(frames[i].view()), threads)
+                if not s:
+                    raise RuntimeError("oracle_awb_measure failed")
+                stats[i] = s
+            for i in sequence:
+                oracle().oracle_awb_apply(self._h, stats[i])
+        finally:
+            for s in stats.values():
+                oracle().oracle_awb_free(s)
 
 
 class RefPipeline:

@@ -4,6 +4,7 @@ and against closed forms the mathematics fixes (the reference's host tests,
 modules/isp/tests/host/cpu_reference_test.cpp, and the closed-form checks of
 its GPU suite, modules/isp/tests/pipeline_test.cpp)."""
 import ctypes as C
+import dataclasses
 import hashlib
 import itertools
 import json
@@ -112,6 +113,42 @@ def test_oracle_threads_match_reference():
         a = OraclePipeline(params).process(frame, out_format, True, (10_000_000, 2.0), threads=threads)
         assert np.array_equal(a.image, b.image) and np.array_equal(a.histogram, b.histogram)
         assert np.array_equal(a.stats.view(np.uint32), b.stats.view(np.uint32))
+
+
+@pytest.mark.parametrize("wb", [_abi.WB_GRAY_WORLD, _abi.WB_WHITE_PATCH])
+def test_white_balance_replay_matches_full_frames(wb):
+    """The replay of a frame sequence's white balance (statistics measured once per distinct frame,
+    then applied in order) leaves the state that running every frame leaves, bit for bit, and the
+    next frame then processes identically: smoothing 0.5 over the sequence 0 1 2 0 1 2 0."""
+    scene, params, out_format = cases.config2(192, 108, seed=5)
+    params = dataclasses.replace(params, white_balance=wb, awb_smoothing=0.5)
+    frames = []
+    for k in range(3):
+        scene.seed = 5 + 1000 * k
+        frames.append(isp.render_scene(scene))
+    sequence = [0, 1, 2, 0, 1, 2, 0]
+    full = OraclePipeline(params)
+    for i in sequence:
+        full.process(frames[i], out_format, True, (10_000_000, 2.0))
+    replay = OraclePipeline(params)
+    replay.replay_white_balance(frames, sequence, threads=2)
+    g_full, v_full = full.awb_state()
+    g_replay, v_replay = replay.awb_state()
+    assert v_full == v_replay == 1
+    assert np.array_equal(np.float32(g_full).view(np.uint32), np.float32(g_replay).view(np.uint32))
+    assert g_full != OraclePipeline(params).awb_state()[0]  # the sequence moved the state
+    a = full.process(frames[1], out_format, True, (10_000_000, 2.0))
+    b = replay.process(frames[1], out_format, True, (10_000_000, 2.0))
+    assert np.array_equal(a.image, b.image) and np.array_equal(a.histogram, b.histogram)
+    assert np.array_equal(a.stats.view(np.uint32), b.stats.view(np.uint32))
+    if oracle_lib.ref_available():
+        ref = oracle_lib.RefPipeline(params)
+        for i in sequence:
+            ref.process(frames[i], out_format, True, (10_000_000, 2.0))
+        g = (C.c_float * 3)()
+        v = C.c_int()
+        oracle_lib.ref().ref_awb_state(ref._h, g, C.byref(v))
+        assert np.array_equal(np.float32(list(g)).view(np.uint32), np.float32(g_replay).view(np.uint32))
 
 
 def test_mirror_brute_force():
