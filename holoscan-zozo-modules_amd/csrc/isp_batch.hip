@@ -267,6 +267,7 @@ struct isp_batch {
     for (const isp_batch_frame& fr : frames) {
       check(hipEventRecord(fr.pipeline->done, stream), "hipEventRecord");
       fr.pipeline->queued = true;
+      fr.pipeline->last_stream = stream;
     }
   }
 
@@ -362,7 +363,7 @@ isp_status isp_batch_create(isp_batch** out) {
     check(hipMalloc(&b->d_control, sizeof(BatchControl)), "hipMalloc");
     check(hipMemset(b->d_control, 0, sizeof(BatchControl)), "hipMemset");
     check(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking), "hipStreamCreate");
-    check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&b->done, kOrderingEventFlags), "hipEventCreate");
     check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     *out = b.release();
   });

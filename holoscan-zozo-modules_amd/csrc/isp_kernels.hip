@@ -1740,6 +1740,11 @@ main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows
     main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign, main_waves_per_eu<kFmt, kSharpen>() <= 4>(
         f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
   }
+  // A frame with no histograms, statistics, exposure state or gray-world state
+  // to finish has nothing to count: its blocks end with their tiles (the
+  // completion chain's serial device atomics take several microseconds, most
+  // of a small frame's time).
+  if (!c.want_stats && !c.want_hist && !gray_world) return;
   const unsigned int token = main_flush<kBlock>(&s->main, fl, c, acc);
   if (!complete_grid(s->main.done, &s->main.top, token, &fl.flag)) return;
   main_finish<kBlock>(f, &s->main, fl, c);

@@ -53,7 +53,7 @@ isp_status isp_create(const isp_params* params, uint8_t backend, isp_pipeline** 
     // earlier keeps its address); feedback starts switched off.
     check(hipMalloc(&p->d_ae, sizeof(AeState)), "hipMalloc");
     check(hipMemset(p->d_ae, 0, sizeof(AeState)), "hipMemset");
-    check(hipEventCreateWithFlags(&p->done, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&p->done, kOrderingEventFlags), "hipEventCreate");
     std::vector<float> padded(p->lut);
     padded.push_back(p->lut.back() - 1.0F);
     upload(padded.data(), padded.size() * sizeof(float), p->d_lut);
@@ -94,7 +94,11 @@ isp_status isp_process(isp_pipeline* pipeline, const isp_raw_view* raw, const is
     const hipStream_t s = static_cast<hipStream_t>(stream);
     // Clear an error an earlier, unrelated call left behind (cuda_pipeline.cu:301-303).
     static_cast<void>(hipGetLastError());
-    if (pipeline->queued) check(hipStreamWaitEvent(s, pipeline->done, 0), "hipStreamWaitEvent");
+    // The previous frame first (cuda_pipeline.cu:303-309); on the stream it was
+    // queued on, the stream's own order already gives that, without a wait packet.
+    if (pipeline->queued && s != pipeline->last_stream) {
+      check(hipStreamWaitEvent(s, pipeline->done, 0), "hipStreamWaitEvent");
+    }
     const isp_params& p = pipeline->params.c;
     const bool timing = pipeline->timing;
     const bool awb = p.white_balance != ISP_WB_MANUAL;
@@ -105,6 +109,7 @@ isp_status isp_process(isp_pipeline* pipeline, const isp_raw_view* raw, const is
     if (timing) pipeline->mark(s);
     check(hipEventRecord(pipeline->done, s), "hipEventRecord");
     pipeline->queued = true;
+    pipeline->last_stream = s;
   });
 }
 

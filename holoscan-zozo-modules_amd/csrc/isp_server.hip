@@ -351,7 +351,7 @@ isp_status isp_server_create(const isp_server_config* config, isp_server** out) 
     s->slots.resize(config->launch_slots);
     for (Slot& slot : s->slots) {
       check(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking), "hipStreamCreate");
-      check(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming), "hipEventCreate");
+      check(hipEventCreateWithFlags(&slot.done, kOrderingEventFlags), "hipEventCreate");
       if (config->per_frame == 0) {
         if (isp_batch_create(&slot.batch) != ISP_OK) throw runtime_failure(isp_last_error());
         if (isp_batch_set_schedule(slot.batch, &config->schedule) != ISP_OK) throw invalid_argument(isp_last_error());
