@@ -291,6 +291,28 @@ struct isp_server {
     return true;
   }
 
+  // Admission in the submitting thread (the mutex held): while a launch slot is
+  // free, waiting frames go to the GPU now, not when the scheduling thread next
+  // wakes up (a sleeping thread's wake-up takes tens of microseconds; at low
+  // load a frame's whole latency is under a hundred). The scheduling thread
+  // still retires the launches. A HIP failure here stops the server, as one in
+  // the scheduling thread does.
+  void admit_here() {
+    if (stop || !failure.empty()) return;
+    int previous = -1;
+    if (hipGetDevice(&previous) != hipSuccess) return;  // left to the scheduling thread
+    if (previous != device && hipSetDevice(device) != hipSuccess) return;
+    try {
+      for (Slot& slot : slots) {
+        if (!slot.busy && !waiting.empty()) launch(slot);
+      }
+    } catch (const std::exception& e) {
+      failure = e.what();
+      stop = true;
+    }
+    if (previous != device) static_cast<void>(hipSetDevice(previous));
+  }
+
   void shutdown() noexcept {
     {
       std::lock_guard<std::mutex> g(mutex);
@@ -462,6 +484,7 @@ isp_status isp_server_submit(isp_server* server, int32_t camera, const void* raw
       ++server->epoch;
       ++server->counters.submitted;
       server->counters.queued = static_cast<int64_t>(server->waiting.size());
+      server->admit_here();
     }
     server->wake.notify_one();
   });
