@@ -247,7 +247,8 @@ def batched_vs_per_frame(ticks):
     batched["schedule"] = {"tile_height": [list(r) for r in chosen.tile_height],
                            "tiles_per_item": [list(r) for r in chosen.tiles_per_item],
                            "awb_tiles_per_item": chosen.awb_tiles_per_item, "blocks_per_cu": chosen.blocks_per_cu,
-                           "order": chosen.order, "queue": chosen.queue, "graph": chosen.graph}
+                           "order": chosen.order, "queue": chosen.queue, "graph": chosen.graph,
+                           "per_frame_kernels": chosen.per_frame}
     return {"workload": "16 streams: 4 x 1280x720 BGGR RAW10, 8 x 1920x1200 RGGB RAW12, 4 x 3840x2160 GBRG u16 "
                         "14-bit; gray-world, bilinear/Malvar alternating, RGB8 + histograms + stats",
             "pixels_per_tick": pixels, "ticks": ticks, "per_frame_launches": single, "batched": batched,
@@ -306,7 +307,7 @@ def large_frames(frames, ticks):
     return {"workload": f"{frames} x 8192x6144 RGGB u16 16-bit, BL 1024, white patch, Malvar, sharpen 0.5, CCM, "
                         "65536-entry sRGB table, RGB16 + 4x1024 histograms + stats",
             "algorithmic_bytes_per_batch": algo, "per_frame_launches": single, "batched": batched,
-            "batched_queue": chosen.queue}
+            "batched_queue": chosen.queue, "batched_per_frame_kernels": chosen.per_frame}
 
 
 def served(seconds, per_frame, device_source, saturate=False, fps=30.0):

@@ -34,6 +34,7 @@ void schedule_defaults(isp_batch_schedule* s) {
   s->order = ISP_BATCH_ORDER_LARGEST_FIRST;
   s->queue = ISP_BATCH_QUEUE_DYNAMIC;
   s->graph = 1;
+  s->per_frame = 0;
 }
 
 void validate_schedule(const isp_batch_schedule& s) {
@@ -58,6 +59,7 @@ void validate_schedule(const isp_batch_schedule& s) {
     throw invalid_argument("schedule.queue is not an ISP_BATCH_QUEUE");
   }
   if (s.graph != 0 && s.graph != 1) throw invalid_argument("schedule.graph must be 0 or 1");
+  if (s.per_frame != 0 && s.per_frame != 1) throw invalid_argument("schedule.per_frame must be 0 or 1");
 }
 
 bool aligned16(const void* p, long long pitch) {
@@ -80,6 +82,7 @@ struct isp_batch {
   bool dirty = false;    // the host table is newer than the device's
   bool sharpen = false;  // a frame sharpens: the batch kernel with the larger LDS
   bool wide = true;      // every frame runs 32-row tiles: the 512-thread kernels
+  int uniform = -1;      // every frame runs this variant with 32-row tiles: its kernel alone
   // Frames whose tone table starts at -0.0 (FrameDesc::zero_sign): they run the
   // per-frame kernels, which have the form that keeps every zero's sign, in the
   // same launch sequence, ahead of the batch kernels; the batch kernels carry
@@ -123,7 +126,7 @@ struct isp_batch {
   }
 
   int resident_per_cu() const {
-    const int fit = batch_main_blocks_per_cu(sharpen, wide);
+    const int fit = batch_main_blocks_per_cu(sharpen, wide, uniform);
     return schedule.blocks_per_cu > 0 ? std::min(schedule.blocks_per_cu, fit) : fit;
   }
 
@@ -139,6 +142,9 @@ struct isp_batch {
     t.queue = schedule.queue;
     sharpen = false;
     wide = true;
+    uniform = -1;
+    int first_variant = -1;
+    bool one_variant = true;
     solo.clear();
     std::vector<double> cost(n);
     for (int i = 0; i < n; ++i) {
@@ -151,7 +157,7 @@ struct isp_batch {
       const bool malvar = c.demosaic == ISP_DEMOSAIC_MALVAR;
       const bool sharpen = c.sharpen > 0.0F;
       const int tile_h = schedule.tile_height[fr.raw.format][malvar ? 1 : 0];
-      if (b.f.zero_sign != 0) {
+      if (b.f.zero_sign != 0 || schedule.per_frame != 0) {
         // No work items in the batch passes: launch() runs the frame by itself.
         solo.push_back(i);
         cost[i] = 0.0;
@@ -160,6 +166,8 @@ struct isp_batch {
       this->sharpen = this->sharpen || sharpen;
       wide = wide && tile_h == 32;
       b.variant = fr.raw.format | (malvar ? 4 : 0) | (sharpen ? 8 : 0) | (tile_h == 16 ? 16 : 0);
+      if (first_variant < 0) first_variant = b.variant;
+      one_variant = one_variant && b.variant == first_variant;
       b.tiles_x = ceil_div(fr.raw.width, kBatchTileW);
       b.tiles = b.tiles_x * ceil_div(fr.raw.height, tile_h);
       b.tiles_per_item = schedule.tiles_per_item[fr.raw.format][malvar ? 1 : 0];
@@ -174,6 +182,7 @@ struct isp_batch {
       b.out_aligned = aligned16(fr.out.data, static_cast<long long>(fr.out.pitch)) ? 1 : 0;
       cost[i] = static_cast<double>(fr.raw.width) * fr.raw.height * (sharpen ? 1.6 : 1.0) * (malvar ? 1.2 : 1.0);
     }
+    if (wide && one_variant && first_variant >= 0) uniform = first_variant;
     // A small launch gets smaller work items, so that every resident block has
     // work: the items per pass are kept at twice the blocks or more when the
     // tiles allow it, never above the schedule's item size.
@@ -240,7 +249,7 @@ struct isp_batch {
         launch_main(b.f, b.s, p->params.c.demosaic == ISP_DEMOSAIC_MALVAR, p->params.c.sharpen > 0.0F, p->launch, st);
       }
       if (awb_blocks > 0) launch_batch_awb(d_table, d_control, awb_blocks, st);
-      if (h_table->main_items > 0) launch_batch_main(d_table, d_control, main_blocks, sharpen, wide, st);
+      if (h_table->main_items > 0) launch_batch_main(d_table, d_control, main_blocks, sharpen, wide, uniform, st);
     };
     if (schedule.graph != 0) {
       if (exec == nullptr || graph_grid != main_blocks * 65536 + awb_blocks) {
@@ -298,7 +307,7 @@ std::string schedule_text(const isp_batch_schedule& s) {
   std::ostringstream o;
   for (const auto& row : s.tile_height) o << row[0] << ' ' << row[1] << ' ';
   for (const auto& row : s.tiles_per_item) o << row[0] << ' ' << row[1] << ' ';
-  o << s.awb_tiles_per_item << ' ' << s.blocks_per_cu << ' ' << s.order << ' ' << s.queue;
+  o << s.awb_tiles_per_item << ' ' << s.blocks_per_cu << ' ' << s.order << ' ' << s.queue << ' ' << s.per_frame;
   return o.str();
 }
 
@@ -309,6 +318,7 @@ bool parse_schedule(const std::string& text, isp_batch_schedule* s) {
   for (auto& row : r.tiles_per_item) in >> row[0] >> row[1];
   in >> r.awb_tiles_per_item >> r.blocks_per_cu >> r.order >> r.queue;
   if (!in) return false;
+  if (!(in >> r.per_frame)) r.per_frame = 0;  // entries written before the field existed
   r.graph = s->graph;
   try {
     validate_schedule(r);
@@ -531,13 +541,20 @@ isp_status isp_batch_tune(isp_batch* batch, isp_stream_t stream, const char* cac
         consider(c);
       }
       {
-        const int fit = batch_main_blocks_per_cu(batch->sharpen, batch->wide);
+        const int fit = batch_main_blocks_per_cu(batch->sharpen, batch->wide, batch->uniform);
         for (int bp : {1, 2}) {
           if (bp >= fit || bp == best.blocks_per_cu) continue;
           isp_batch_schedule c = best;
           c.blocks_per_cu = bp;
           consider(c);
         }
+      }
+      // Last, the per-frame kernels one frame after another: few large frames fill
+      // the GPU one at a time, and their kernels hold one frame's tables in the caches.
+      {
+        isp_batch_schedule c = best;
+        c.per_frame = 1;
+        consider(c);
       }
     } catch (...) {
       static_cast<void>(hipEventDestroy(t0));

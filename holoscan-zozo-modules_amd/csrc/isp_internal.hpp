@@ -217,9 +217,11 @@ void launch_epilogue(const isp_epilogue_case* cases, float* out, int count, void
 // without.
 int batch_awb_blocks_per_cu();
 // wide: the 512-thread kernels, for batches whose frames all run 32-row tiles.
-int batch_main_blocks_per_cu(bool sharpen, bool wide);
+// uniform >= 0: every frame runs that variant (BatchFrame::variant, 32-row tiles),
+// and a kernel with that tile body alone runs the batch.
+int batch_main_blocks_per_cu(bool sharpen, bool wide, int uniform = -1);
 void launch_batch_awb(const BatchTable* table, BatchControl* control, int grid, void* stream);
 void launch_batch_main(const BatchTable* table, BatchControl* control, int grid, bool sharpen, bool wide,
-                       void* stream);
+                       int uniform, void* stream);
 
 }  // namespace isp_amd

@@ -62,6 +62,7 @@
 #include <algorithm>
 #include <climits>
 #include <map>
+#include <utility>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -1968,10 +1969,17 @@ __global__ void __launch_bounds__(kThreads) batch_awb_kernel(const BatchTable* _
 // batch without them runs a lighter kernel. kBlock: 512-thread blocks (16 waves
 // per CU, as the per-frame kernel) run 32-row tiles only; a batch with 16-row
 // tiles runs the 256-thread kernels.
-template <bool kSharpen, int kBlock>
+// kOnly >= 0: every frame of the batch runs the tile body of variant kOnly
+// (32-row tiles), which is then the only one compiled in: no switch over the
+// variants, and the registers and LDS of that body alone.
+template <bool kSharpen, int kBlock, int kOnly = -1>
 __device__ __forceinline__ void batch_main_body(const BatchTable* __restrict__ table, BatchControl* ctl) {
+  constexpr int kOnlyFmt = kOnly & 3;
+  constexpr bool kOnlyMalvar = (kOnly & 4) != 0;
   __shared__ MainFrameLds fl;
-  __shared__ __attribute__((aligned(16))) unsigned char tile_lds[batch_tile_lds<kSharpen, kBlock>()];
+  __shared__ __attribute__((aligned(16))) unsigned char
+      tile_lds[kOnly >= 0 ? sizeof(MainTileLds<kOnlyFmt, kOnlyMalvar, kSharpen, 32, kBlock>)
+                          : batch_tile_lds<kSharpen, kBlock>()];
   __shared__ int slot;
   BatchQueue q{&ctl->main_next, &ctl->main_exited, table->main_items, table->queue, &slot};
   const int count = table->count;
@@ -2013,13 +2021,17 @@ __device__ __forceinline__ void batch_main_body(const BatchTable* __restrict__ t
       if (threadIdx.x == 0) mc = begun;
       __syncthreads();
     }
-    switch (b.variant) {
-      ISP_BATCH_MAIN_CASES(ISP_RAW_U8)
-      ISP_BATCH_MAIN_CASES(ISP_RAW_U16)
-      ISP_BATCH_MAIN_CASES(ISP_RAW_RAW10)
-      ISP_BATCH_MAIN_CASES(ISP_RAW_RAW12)
-      default:
-        break;
+    if constexpr (kOnly >= 0) {
+      batch_main_item<kOnlyFmt, kOnlyMalvar, kSharpen, 32, kBlock>(b, mc, fl, tile_lds, local, acc);
+    } else {
+      switch (b.variant) {
+        ISP_BATCH_MAIN_CASES(ISP_RAW_U8)
+        ISP_BATCH_MAIN_CASES(ISP_RAW_U16)
+        ISP_BATCH_MAIN_CASES(ISP_RAW_RAW10)
+        ISP_BATCH_MAIN_CASES(ISP_RAW_RAW12)
+        default:
+          break;
+      }
     }
     ++items_done;
   }
@@ -2051,6 +2063,26 @@ batch_main_kernel_sharpen_wide(const BatchTable* __restrict__ table, BatchContro
 __global__ void __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(4)))
 batch_main_kernel_plain_wide(const BatchTable* __restrict__ table, BatchControl* ctl) {
   batch_main_body<false, kWideBlock>(table, ctl);
+}
+// A batch whose frames all run one variant with 32-row tiles (variant_of(), no
+// 16-row bit): that tile body alone, at the per-frame kernel's registers.
+template <int kVariant>
+__global__ void __launch_bounds__(kWideBlock)
+__attribute__((amdgpu_waves_per_eu(main_waves_per_eu<kVariant & 3, (kVariant & 8) != 0>())))
+batch_main_kernel_uniform(const BatchTable* __restrict__ table, BatchControl* ctl) {
+  batch_main_body<(kVariant & 8) != 0, kWideBlock, kVariant>(table, ctl);
+}
+
+// The uniform kernel of a variant (0..15, 32-row tiles).
+using BatchKernel = void (*)(const BatchTable*, BatchControl*);
+template <int... kV>
+constexpr BatchKernel uniform_kernel_of(int variant, std::integer_sequence<int, kV...>) {
+  BatchKernel k = nullptr;
+  ((k = kV == variant ? batch_main_kernel_uniform<kV> : k), ...);
+  return k;
+}
+BatchKernel uniform_kernel(int variant) {
+  return uniform_kernel_of(variant, std::make_integer_sequence<int, 16>{});
 }
 
 // ---- verification: the statistics epilogue on the device ---------------------
@@ -2194,7 +2226,8 @@ void launch_epilogue(const isp_epilogue_case* cases, float* out, int count, void
   check(hipGetLastError(), "epilogue_kernel launch");
 }
 
-int batch_main_blocks_per_cu(bool sharpen, bool wide) {
+int batch_main_blocks_per_cu(bool sharpen, bool wide, int uniform) {
+  if (uniform >= 0) return blocks_per_cu(uniform_kernel(uniform), kWideBlock);
   if (wide) {
     return sharpen ? blocks_per_cu(batch_main_kernel_sharpen_wide, kWideBlock)
                    : blocks_per_cu(batch_main_kernel_plain_wide, kWideBlock);
@@ -2211,9 +2244,11 @@ void launch_batch_awb(const BatchTable* table, BatchControl* control, int grid, 
 }
 
 void launch_batch_main(const BatchTable* table, BatchControl* control, int grid, bool sharpen, bool wide,
-                       void* stream) {
+                       int uniform, void* stream) {
   const hipStream_t st = static_cast<hipStream_t>(stream);
-  if (wide) {
+  if (uniform >= 0) {
+    hipLaunchKernelGGL(uniform_kernel(uniform), dim3(grid), dim3(kWideBlock), 0, st, table, control);
+  } else if (wide) {
     if (sharpen) {
       hipLaunchKernelGGL(batch_main_kernel_sharpen_wide, dim3(grid), dim3(kWideBlock), 0, st, table, control);
     } else {

@@ -90,7 +90,7 @@ class BatchFrame(C.Structure):
 class BatchSchedule(C.Structure):
     _fields_ = [("tile_height", (C.c_int32 * 2) * 4), ("tiles_per_item", (C.c_int32 * 2) * 4),
                 ("awb_tiles_per_item", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("order", C.c_int32), ("queue", C.c_int32), ("graph", C.c_int32)]
+                ("order", C.c_int32), ("queue", C.c_int32), ("graph", C.c_int32), ("per_frame", C.c_int32)]
 
 
 class ServerConfig(C.Structure):

@@ -293,10 +293,13 @@ typedef struct isp_batch_schedule {
   int32_t order;              /* ISP_BATCH_ORDER_*: which frames' work the queue lists first */
   int32_t queue;              /* ISP_BATCH_QUEUE_* */
   int32_t graph;              /* 1: capture the launches as a hipGraph once and replay it */
+  int32_t per_frame;          /* 1: every frame runs the per-frame kernels, one frame after another
+                                 in the batch's launch (and graph); the other fields then do not apply */
 } isp_batch_schedule;
 
 /* The default schedule: 32-row tiles, 4 tiles per item, 8 white-balance tiles
- * per item, full occupancy, largest frames first, the dynamic queue, graph replay. */
+ * per item, full occupancy, largest frames first, the dynamic queue, graph replay,
+ * the batch kernels. */
 void isp_batch_schedule_init(isp_batch_schedule* schedule);
 isp_status isp_batch_create(isp_batch** out);
 /* Waits for the batch's launches, then frees. */

@@ -74,7 +74,7 @@ def test_batch_schedule_defaults_without_a_device():
     s = isp.default_schedule()
     assert [list(r) for r in s.tile_height] == [[32, 32]] * 4
     assert [list(r) for r in s.tiles_per_item] == [[4, 4]] * 4
-    assert (s.awb_tiles_per_item, s.blocks_per_cu, s.order, s.queue, s.graph) == (8, 0, 0, 0, 1)
+    assert (s.awb_tiles_per_item, s.blocks_per_cu, s.order, s.queue, s.graph, s.per_frame) == (8, 0, 0, 0, 1, 0)
 
 
 def test_batch_create_without_a_device_fails_cleanly():

@@ -159,6 +159,8 @@ SCHEDULES = [
     dict(tiles_per_item=1, awb_tiles_per_item=1),
     dict(tiles_per_item=32, awb_tiles_per_item=64, order=_abi.BATCH_ORDER_AS_GIVEN),
     dict(blocks_per_cu=1, tile16=True, graph=0),
+    dict(per_frame=1),
+    dict(per_frame=1, graph=0),
 ]
 
 
@@ -183,6 +185,41 @@ def test_every_schedule_gives_the_same_bits(torch_gpu, options):
     for t in range(2):
         for i in range(len(specs)):
             compare(f"tick {t} stream {i}", got[t][i], want[t][i])
+
+
+def one_variant_specs(fmt, bits, dem, sharpen):
+    """Frames that all run one tile body (raw format, demosaic, sharpening, 32-row tiles), at
+    several sizes, patterns and settings: the batch then runs the kernel of that body alone."""
+    rows = [(320, 180, _abi.RGGB, _abi.WB_GRAY_WORLD, _abi.OUT_RGBA8), (130, 66, _abi.GBRG, _abi.WB_WHITE_PATCH, _abi.OUT_RGB8),
+            (67, 35, _abi.GRBG, _abi.WB_MANUAL, _abi.OUT_RGB16), (258, 100, _abi.BGGR, _abi.WB_GRAY_WORLD, _abi.OUT_RGB32F)]
+    specs = []
+    for i, (w, h, pattern, wb, out) in enumerate(rows):
+        black = 16.0 if bits == 8 else 64.0
+        sc = cases.baseline_scene(w, h, fmt, bits, pattern, black, 700 + i)
+        p = isp.IspParams(black_level=black, lens_shading=isp.scene_lens_shading(sc, 9, 7) if i % 2 == 0 else None,
+                          white_balance=wb, wb_gains=(1.8, 1.0, 1.5), demosaic=dem, sharpen=sharpen,
+                          ccm=cases.stream_ccm(i), histogram_bins=1024 if i == 2 else 256)
+        specs.append((sc, p, out))
+    return specs
+
+
+@pytest.mark.parametrize("fmt,bits,dem,sharpen", [(_abi.RAW_U16, 12, _abi.MALVAR, 0.5), (_abi.RAW_RAW12, 12, _abi.MALVAR, 0.5),
+                                                  (_abi.RAW_RAW10, 10, _abi.BILINEAR, 0.0), (_abi.RAW_U8, 8, _abi.BILINEAR, 0.5),
+                                                  (_abi.RAW_U16, 14, _abi.BILINEAR, 0.0)],
+                         ids=["u16-malvar-sharpen", "raw12-malvar-sharpen", "raw10-bilinear", "u8-bilinear-sharpen",
+                              "u16-bilinear"])
+def test_one_variant_batch_matches_per_frame_and_oracle(torch_gpu, fmt, bits, dem, sharpen):
+    """A batch of one tile body runs that body's own kernel; results equal the per-frame runs and
+    the oracle over two ticks, eager and replayed."""
+    specs = one_variant_specs(fmt, bits, dem, sharpen)
+    single = run_single(specs, ticks=2)
+    oracle = run_oracle(specs, ticks=2)
+    for graph in (0, 1):
+        batched = run_batch(specs, make_schedule(graph=graph), ticks=2)
+        for t in range(2):
+            for i in range(len(specs)):
+                compare(f"graph {graph} tick {t} stream {i} vs per-frame", batched[t][i], single[t][i])
+                compare(f"graph {graph} tick {t} stream {i} vs oracle", batched[t][i], oracle[t][i])
 
 
 def test_graph_replays_with_other_work_between(torch_gpu):

@@ -132,5 +132,7 @@ def test_kernels_make_no_calls_and_use_no_scratch():
         scratch = int(re.search(r"\.amdhsa_private_segment_fixed_size (\d+)", body).group(1))
         # The 512-thread batch kernels hold every tile variant in one function at 128
         # registers; a few per-thread constants spill once per launch (at most 64 bytes).
-        limit = 64 if "_wide" in name else 0
+        # The one-variant batch kernels add the queue and frame switch to a tile body
+        # at the per-frame kernel's registers: a word or a few (at most 32 bytes).
+        limit = 64 if "_wide" in name else 32 if "uniform" in name else 0
         assert scratch <= limit, f"{name}: {scratch} bytes of scratch"
