@@ -208,8 +208,9 @@ struct LaunchConfig {
   int awb_blocks = 3;  // white balance kernel: resident blocks per CU at most
 };
 void launch_awb(const FrameDesc& f, DeviceScratch* scratch, const LaunchConfig& cfg, void* stream);
+// `done`, when not null, is a hipEvent_t that completes with the main kernel's dispatch.
 void launch_main(const FrameDesc& f, DeviceScratch* scratch, bool malvar, bool sharpen,
-                 const LaunchConfig& cfg, void* stream);
+                 const LaunchConfig& cfg, void* stream, void* done = nullptr);
 // The statistics epilogue over `count` cases in device memory (verification).
 void launch_epilogue(const isp_epilogue_case* cases, float* out, int count, void* stream);
 // The batch kernels' resident blocks per CU and their launches (grid blocks);

@@ -57,6 +57,7 @@
 // Every float expression keeps the reference's operation order
 // (isp_math.hpp:121-305) and the library is built with -ffp-contract=off, so
 // the output is bit for bit the reference CPU backend's.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -2146,7 +2147,8 @@ template <bool kSharpen>
 constexpr int kMainBlock = 512;
 
 template <int kFmt, bool kMalvar, bool kSharpen>
-void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream) {
+void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream,
+                   hipEvent_t done) {
   constexpr int kBlock = kMainBlock<kSharpen>;
   auto kernel = f.zero_sign != 0 ? main_kernel<kFmt, kMalvar, kSharpen, kMainTileH, kBlock, true>
                                   : main_kernel<kFmt, kMalvar, kSharpen, kMainTileH, kBlock, false>;
@@ -2154,18 +2156,27 @@ void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg
   const int tiles_y = (f.height + kMainTileH - 1) / kMainTileH;
   const int tiles = tiles_x * tiles_y;
   const int grid = std::max(1, std::min(tiles, cfg.cu_count * blocks_per_cu(kernel, kBlock)));
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, stream, f, s, tiles_x, tiles, rows_aligned(f) ? 1 : 0,
-                     out_aligned(f) ? 1 : 0);
+  const int rows = rows_aligned(f) ? 1 : 0;
+  const int outs = out_aligned(f) ? 1 : 0;
+  if (done != nullptr) {
+    // The event completes with the dispatch itself: no marker packet of its own.
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, stream, nullptr, done, 0, f, s, tiles_x, tiles, rows,
+                          outs);
+  } else {
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, stream, f, s, tiles_x, tiles, rows, outs);
+  }
   check(hipGetLastError(), "main_kernel launch");
 }
 
 template <int kFmt>
 void launch_main_f(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen, const LaunchConfig& cfg,
-                   hipStream_t st) {
+                   hipStream_t st, hipEvent_t done) {
   if (malvar) {
-    sharpen ? launch_main_t<kFmt, true, true>(f, s, cfg, st) : launch_main_t<kFmt, true, false>(f, s, cfg, st);
+    sharpen ? launch_main_t<kFmt, true, true>(f, s, cfg, st, done)
+            : launch_main_t<kFmt, true, false>(f, s, cfg, st, done);
   } else {
-    sharpen ? launch_main_t<kFmt, false, true>(f, s, cfg, st) : launch_main_t<kFmt, false, false>(f, s, cfg, st);
+    sharpen ? launch_main_t<kFmt, false, true>(f, s, cfg, st, done)
+            : launch_main_t<kFmt, false, false>(f, s, cfg, st, done);
   }
 }
 
@@ -2201,20 +2212,21 @@ void launch_awb(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, v
 }
 
 void launch_main(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen, const LaunchConfig& cfg,
-                 void* stream) {
+                 void* stream, void* done) {
   const hipStream_t st = static_cast<hipStream_t>(stream);
+  const hipEvent_t ev = static_cast<hipEvent_t>(done);
   switch (f.raw_format) {
     case ISP_RAW_U8:
-      launch_main_f<ISP_RAW_U8>(f, s, malvar, sharpen, cfg, st);
+      launch_main_f<ISP_RAW_U8>(f, s, malvar, sharpen, cfg, st, ev);
       break;
     case ISP_RAW_U16:
-      launch_main_f<ISP_RAW_U16>(f, s, malvar, sharpen, cfg, st);
+      launch_main_f<ISP_RAW_U16>(f, s, malvar, sharpen, cfg, st, ev);
       break;
     case ISP_RAW_RAW10:
-      launch_main_f<ISP_RAW_RAW10>(f, s, malvar, sharpen, cfg, st);
+      launch_main_f<ISP_RAW_RAW10>(f, s, malvar, sharpen, cfg, st, ev);
       break;
     default:
-      launch_main_f<ISP_RAW_RAW12>(f, s, malvar, sharpen, cfg, st);
+      launch_main_f<ISP_RAW_RAW12>(f, s, malvar, sharpen, cfg, st, ev);
       break;
   }
 }

@@ -105,9 +105,11 @@ isp_status isp_process(isp_pipeline* pipeline, const isp_raw_view* raw, const is
     if (timing && awb) pipeline->mark(s);
     if (awb) launch_awb(f, pipeline->d_scratch, pipeline->launch, s);
     if (timing) pipeline->mark(s);
-    launch_main(f, pipeline->d_scratch, p.demosaic == ISP_DEMOSAIC_MALVAR, p.sharpen > 0.0F, pipeline->launch, s);
+    // The frame-ordering event completes with the main kernel's dispatch
+    // (cuda_pipeline.cu:359 records it after the last kernel instead).
+    launch_main(f, pipeline->d_scratch, p.demosaic == ISP_DEMOSAIC_MALVAR, p.sharpen > 0.0F, pipeline->launch, s,
+                pipeline->done);
     if (timing) pipeline->mark(s);
-    check(hipEventRecord(pipeline->done, s), "hipEventRecord");
     pipeline->queued = true;
     pipeline->last_stream = s;
   });
