@@ -1251,6 +1251,39 @@ __device__ __forceinline__ int output_row(int tid) {
 
 // XCDs of the chip; blocks are dispatched to them in turn.
 constexpr int kXcds = 8;
+static_assert((kXcds & (kXcds - 1)) == 0, "a power of two");
+
+// A block's share of `total` items when the grid's blocks split into one group
+// per XCD (block b in group b % 8, the XCD it runs on) and each group sweeps
+// one contiguous range: the block's first item, the end of its group's range,
+// and the stride between its items. In unsigned 32-bit arithmetic, and with
+// eight groups (every grid of eight blocks or more) the divisions are shifts:
+// every wave runs this, and a 64-bit division is some 150 scalar instructions.
+// total x 8 stays far below 2^32: frames are at most 16384 x 16384 samples
+// (validate_raw), 2^17 tiles, and a batch holds at most 64 of them.
+struct XcdShare {
+  int first;
+  int end;
+  int stride;
+};
+__device__ __forceinline__ XcdShare xcd_share(int total) {
+  const unsigned int g = gridDim.x;
+  const unsigned int b = blockIdx.x;
+  const unsigned int t = static_cast<unsigned int>(total);
+  XcdShare s;
+  if (g >= static_cast<unsigned int>(kXcds)) {
+    constexpr unsigned int kShift = __builtin_ctz(kXcds);
+    const unsigned int x = b & (kXcds - 1);
+    s.first = static_cast<int>(((t * x) >> kShift) + (b >> kShift));
+    s.end = static_cast<int>((t * (x + 1)) >> kShift);
+    s.stride = static_cast<int>((g - x + kXcds - 1) >> kShift);
+  } else {
+    s.first = static_cast<int>(t * b / g);
+    s.end = static_cast<int>(t * (b + 1) / g);
+    s.stride = 1;
+  }
+  return s;
+}
 
 // Waves per SIMD the main pass's registers are held to: four (two 512-thread
 // blocks a CU); six (three blocks) without sharpening for u8 and u16 samples,
@@ -1732,13 +1765,8 @@ main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows
   // Tiles by XCD: blocks are dispatched to the eight XCDs in turn (block b to
   // XCD b % 8), and each XCD's blocks sweep one contiguous band of tiles, so
   // that neighbouring tiles, whose halos overlap, are read through one L2.
-  const int g = static_cast<int>(gridDim.x);
-  const int parts = g < kXcds ? g : kXcds;
-  const int xcd = static_cast<int>(blockIdx.x) % parts;
-  const int members = (g - xcd + parts - 1) / parts;
-  const int band_end = static_cast<int>(static_cast<long long>(tiles) * (xcd + 1) / parts);
-  const int band_begin = static_cast<int>(static_cast<long long>(tiles) * xcd / parts);
-  for (int tile = band_begin + static_cast<int>(blockIdx.x) / parts; tile < band_end; tile += members) {
+  const XcdShare share = xcd_share(tiles);
+  for (int tile = share.first; tile < share.end; tile += share.stride) {
     main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign, main_waves_per_eu<kFmt, kSharpen>() <= 4>(
         f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
   }
@@ -1848,14 +1876,10 @@ struct BatchQueue {
   __device__ __forceinline__ int first() {
     if (mode == ISP_BATCH_QUEUE_STATIC) return static_cast<int>(blockIdx.x);
     if (mode == ISP_BATCH_QUEUE_STATIC_XCD) {
-      const int g = static_cast<int>(gridDim.x);
-      const int parts = g < kXcds ? g : kXcds;
-      const int x = static_cast<int>(blockIdx.x) % parts;
-      const int j = static_cast<int>(blockIdx.x) / parts;
-      step = (g - x + parts - 1) / parts;
-      const int begin = static_cast<int>(static_cast<long long>(total) * x / parts);
-      end = static_cast<int>(static_cast<long long>(total) * (x + 1) / parts);
-      return begin + j < end ? begin + j : total;
+      const XcdShare share = xcd_share(total);
+      step = share.stride;
+      end = share.end;
+      return share.first < end ? share.first : total;
     }
     if (threadIdx.x == 0) *slot = static_cast<int>(blockIdx.x);
     __syncthreads();
