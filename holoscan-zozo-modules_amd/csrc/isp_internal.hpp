@@ -209,8 +209,10 @@ struct LaunchConfig {
 };
 void launch_awb(const FrameDesc& f, DeviceScratch* scratch, const LaunchConfig& cfg, void* stream);
 // `done`, when not null, is a hipEvent_t that completes with the main kernel's dispatch.
+// `plain`: the frame has nothing to finish (no histograms, statistics, exposure
+// state or gray-world state), which lets the kernel do without their LDS.
 void launch_main(const FrameDesc& f, DeviceScratch* scratch, bool malvar, bool sharpen,
-                 const LaunchConfig& cfg, void* stream, void* done = nullptr);
+                 const LaunchConfig& cfg, void* stream, void* done = nullptr, bool plain = false);
 // The statistics epilogue over `count` cases in device memory (verification).
 void launch_epilogue(const isp_epilogue_case* cases, float* out, int count, void* stream);
 // The batch kernels' resident blocks per CU and their launches (grid blocks);

@@ -63,6 +63,7 @@
 #include <algorithm>
 #include <climits>
 #include <map>
+#include <type_traits>
 #include <utility>
 #include <mutex>
 #include <stdexcept>
@@ -946,6 +947,7 @@ constexpr int kHistEntries = 8 * kMaxHistBins;
 // LDS of the main pass: per frame (histograms, lens shading grid,
 // reductions), and per tile (samples, demosaiced ring, lens shading table).
 struct MainFrameLds {
+  static constexpr bool kPlain = false;
   unsigned int hist[kHistEntries];
   float grid[kMaxGrid];
   unsigned long long part[8][4];  // per wave, up to 512 threads
@@ -953,6 +955,13 @@ struct MainFrameLds {
   float gains[3];
   int wb_update;  // gray world: the frame's gains replace the white balance state
   int flag;
+};
+// The same for a frame with nothing to finish (no histograms, statistics or
+// device state): no histograms, which leaves room for more blocks a CU.
+struct MainPlainFrameLds {
+  static constexpr bool kPlain = true;
+  unsigned int hist[1];  // never used
+  float grid[kMaxGrid];
 };
 template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads>
 struct MainTileLds {
@@ -1025,11 +1034,11 @@ __device__ __forceinline__ void main_gray_world_store(DeviceScratch* s, const Ma
 
 // Starts a block's work on a frame. The tile's first barrier orders these LDS
 // writes before any reader.
-template <int kBlock = kThreads>
-__device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains[3], MainFrameLds& fl, MainCtx& c) {
+template <int kBlock = kThreads, typename FL = MainFrameLds>
+__device__ __forceinline__ void main_begin(const FrameDesc& f, const float gains[3], FL& fl, MainCtx& c) {
   const int tid = threadIdx.x;
-  c.want_hist = f.histogram != nullptr;
-  c.want_stats = f.stats != nullptr || (f.ae_state != nullptr && f.ae_state->enabled != 0);
+  c.want_hist = !FL::kPlain && f.histogram != nullptr;
+  c.want_stats = !FL::kPlain && (f.stats != nullptr || (f.ae_state != nullptr && f.ae_state->enabled != 0));
   c.bins = f.bins;
   c.copies = kHistEntries / (4 * c.bins);
   c.copy_shift = __builtin_ctz(static_cast<unsigned int>(c.copies)) + 2;
@@ -1289,10 +1298,12 @@ __device__ __forceinline__ XcdShare xcd_share(int total) {
 // blocks a CU); six (three blocks) without sharpening for u8 and u16 samples,
 // whose tile LDS (52 KB) lets three blocks in and whose bodies fit 80 VGPRs
 // without spilling (the packed formats' unpacking would spill). Three blocks
-// made the 1080p plain frame 8 % faster and the 4K one 7-9 %.
-template <int kFmt, bool kSharpen>
+// made the 1080p plain frame 8 % faster and the 4K one 7-9 %. The plain kernel
+// (256-thread blocks of 16-row tiles, no histograms) fits 80 VGPRs in every
+// format: six blocks a CU.
+template <int kFmt, bool kSharpen, bool kPlain = false>
 constexpr int main_waves_per_eu() {
-  return !kSharpen && (kFmt == ISP_RAW_U8 || kFmt == ISP_RAW_U16) ? 6 : 4;
+  return kPlain || (!kSharpen && (kFmt == ISP_RAW_U8 || kFmt == ISP_RAW_U16)) ? 6 : 4;
 }
 
 // One kTileH x 64 output tile; statistics sums into acc[0..2], saturated
@@ -1302,8 +1313,8 @@ constexpr int main_waves_per_eu() {
 // batch kernels, with every tile body in one function, and the six-wave kernels
 // have none).
 template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock = kThreads, bool kZeroSign = false,
-          bool kSpare = false>
-__device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_, MainFrameLds& fl,
+          bool kSpare = false, typename FL = MainFrameLds>
+__device__ __forceinline__ void main_tile(const FrameDesc& f, const MainCtx& cx_, FL& fl,
                                           MainTileLds<kFmt, kMalvar, kSharpen, kTileH, kBlock>& tl, int tile,
                                           int tiles_x, int fast_rows, int out_aligned, unsigned long long acc[4]) {
   using G = MainGeom<kFmt, kMalvar, kSharpen, kTileH, kBlock>;
@@ -1738,13 +1749,19 @@ __device__ __forceinline__ void main_finish(const FrameDesc& f, MainScratch* m, 
 // kBlock threads per block: 512-thread blocks (one output row per thread) put
 // twice the waves on a CU for the same LDS, with registers held to what
 // main_waves_per_eu() leaves.
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock, bool kZeroSign>
+//
+// kPlain: a frame with nothing to finish (no histograms, statistics, exposure
+// state or gray-world state; the host decides), in 256-thread blocks of 16-row
+// tiles without the histograms' LDS: more, smaller blocks a CU, so that a small
+// frame's tiles spread over the chip in fewer rounds.
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock, bool kZeroSign, bool kPlain = false>
 __global__ void __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(kBlock >= 512 ? main_waves_per_eu<kFmt, kSharpen>() : 1)))
+__attribute__((amdgpu_waves_per_eu(kBlock >= 512 || kPlain ? main_waves_per_eu<kFmt, kSharpen, kPlain>() : 1)))
 main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows, int out_aligned) {
-  __shared__ MainFrameLds fl;
+  using FL = std::conditional_t<kPlain, MainPlainFrameLds, MainFrameLds>;
+  __shared__ FL fl;
   __shared__ MainTileLds<kFmt, kMalvar, kSharpen, kTileH, kBlock> tl;
-  const bool gray_world = f.wb_mode == ISP_WB_GRAY_WORLD;
+  const bool gray_world = !kPlain && f.wb_mode == ISP_WB_GRAY_WORLD;
   float gains[3];
   if (f.wb_mode != ISP_WB_MANUAL && !gray_world) {
 #pragma unroll
@@ -1753,13 +1770,17 @@ main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows
 #pragma unroll
     for (int k = 0; k < 3; ++k) gains[k] = f.wb[k];  // gray world: replaced below
   }
-  if (gray_world) main_gray_world_gains(f, s, fl);
+  if constexpr (!kPlain) {
+    if (gray_world) main_gray_world_gains(f, s, fl);
+  }
   MainCtx c;
   main_begin<kBlock>(f, gains, fl, c);
   __syncthreads();  // the staged grid (and gray world's gains) before any use
-  if (gray_world) {
-    const float g[3] = {fl.gains[0], fl.gains[1], fl.gains[2]};
-    main_set_gains(f, g, c);
+  if constexpr (!kPlain) {
+    if (gray_world) {
+      const float g[3] = {fl.gains[0], fl.gains[1], fl.gains[2]};
+      main_set_gains(f, g, c);
+    }
   }
   unsigned long long acc[4] = {0, 0, 0, 0};
   // Tiles by XCD: blocks are dispatched to the eight XCDs in turn (block b to
@@ -1767,19 +1788,21 @@ main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows
   // that neighbouring tiles, whose halos overlap, are read through one L2.
   const XcdShare share = xcd_share(tiles);
   for (int tile = share.first; tile < share.end; tile += share.stride) {
-    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign, main_waves_per_eu<kFmt, kSharpen>() <= 4>(
+    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign, main_waves_per_eu<kFmt, kSharpen, kPlain>() <= 4>(
         f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
   }
   // A frame with no histograms, statistics, exposure state or gray-world state
   // to finish has nothing to count: its blocks end with their tiles (the
   // completion chain's serial device atomics take several microseconds, most
   // of a small frame's time).
-  if (!c.want_stats && !c.want_hist && !gray_world) return;
-  const unsigned int token = main_flush<kBlock>(&s->main, fl, c, acc);
-  if (!complete_grid(s->main.done, &s->main.top, token, &fl.flag)) return;
-  main_finish<kBlock>(f, &s->main, fl, c);
-  if (gray_world) main_gray_world_store(s, fl);
-  reset_grid(s->main.done, &s->main.top);
+  if constexpr (!kPlain) {
+    if (!c.want_stats && !c.want_hist && !gray_world) return;
+    const unsigned int token = main_flush<kBlock>(&s->main, fl, c, acc);
+    if (!complete_grid(s->main.done, &s->main.top, token, &fl.flag)) return;
+    main_finish<kBlock>(f, &s->main, fl, c);
+    if (gray_world) main_gray_world_store(s, fl);
+    reset_grid(s->main.done, &s->main.top);
+  }
 }
 
 // ---- the batched form ----------------------------------------------------------
@@ -2170,14 +2193,13 @@ constexpr int kMainTileH = 32;
 template <bool kSharpen>
 constexpr int kMainBlock = 512;
 
-template <int kFmt, bool kMalvar, bool kSharpen>
-void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream,
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock, bool kPlain>
+void launch_main_k(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream,
                    hipEvent_t done) {
-  constexpr int kBlock = kMainBlock<kSharpen>;
-  auto kernel = f.zero_sign != 0 ? main_kernel<kFmt, kMalvar, kSharpen, kMainTileH, kBlock, true>
-                                  : main_kernel<kFmt, kMalvar, kSharpen, kMainTileH, kBlock, false>;
+  auto kernel = f.zero_sign != 0 ? main_kernel<kFmt, kMalvar, kSharpen, kTileH, kBlock, true, kPlain>
+                                  : main_kernel<kFmt, kMalvar, kSharpen, kTileH, kBlock, false, kPlain>;
   const int tiles_x = (f.width + 63) / 64;
-  const int tiles_y = (f.height + kMainTileH - 1) / kMainTileH;
+  const int tiles_y = (f.height + kTileH - 1) / kTileH;
   const int tiles = tiles_x * tiles_y;
   const int grid = std::max(1, std::min(tiles, cfg.cu_count * blocks_per_cu(kernel, kBlock)));
   const int rows = rows_aligned(f) ? 1 : 0;
@@ -2192,15 +2214,29 @@ void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg
   check(hipGetLastError(), "main_kernel launch");
 }
 
+// Frames with nothing to finish and no sharpening run the plain kernel: 16-row
+// tiles in 256-thread blocks without the histograms' LDS.
+template <int kFmt, bool kMalvar, bool kSharpen>
+void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream,
+                   hipEvent_t done, bool plain) {
+  if constexpr (!kSharpen) {
+    if (plain) {
+      launch_main_k<kFmt, kMalvar, false, 16, kThreads, true>(f, s, cfg, stream, done);
+      return;
+    }
+  }
+  launch_main_k<kFmt, kMalvar, kSharpen, kMainTileH, kMainBlock<kSharpen>, false>(f, s, cfg, stream, done);
+}
+
 template <int kFmt>
 void launch_main_f(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen, const LaunchConfig& cfg,
-                   hipStream_t st, hipEvent_t done) {
+                   hipStream_t st, hipEvent_t done, bool plain) {
   if (malvar) {
-    sharpen ? launch_main_t<kFmt, true, true>(f, s, cfg, st, done)
-            : launch_main_t<kFmt, true, false>(f, s, cfg, st, done);
+    sharpen ? launch_main_t<kFmt, true, true>(f, s, cfg, st, done, plain)
+            : launch_main_t<kFmt, true, false>(f, s, cfg, st, done, plain);
   } else {
-    sharpen ? launch_main_t<kFmt, false, true>(f, s, cfg, st, done)
-            : launch_main_t<kFmt, false, false>(f, s, cfg, st, done);
+    sharpen ? launch_main_t<kFmt, false, true>(f, s, cfg, st, done, plain)
+            : launch_main_t<kFmt, false, false>(f, s, cfg, st, done, plain);
   }
 }
 
@@ -2236,21 +2272,21 @@ void launch_awb(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, v
 }
 
 void launch_main(const FrameDesc& f, DeviceScratch* s, bool malvar, bool sharpen, const LaunchConfig& cfg,
-                 void* stream, void* done) {
+                 void* stream, void* done, bool plain) {
   const hipStream_t st = static_cast<hipStream_t>(stream);
   const hipEvent_t ev = static_cast<hipEvent_t>(done);
   switch (f.raw_format) {
     case ISP_RAW_U8:
-      launch_main_f<ISP_RAW_U8>(f, s, malvar, sharpen, cfg, st, ev);
+      launch_main_f<ISP_RAW_U8>(f, s, malvar, sharpen, cfg, st, ev, plain);
       break;
     case ISP_RAW_U16:
-      launch_main_f<ISP_RAW_U16>(f, s, malvar, sharpen, cfg, st, ev);
+      launch_main_f<ISP_RAW_U16>(f, s, malvar, sharpen, cfg, st, ev, plain);
       break;
     case ISP_RAW_RAW10:
-      launch_main_f<ISP_RAW_RAW10>(f, s, malvar, sharpen, cfg, st, ev);
+      launch_main_f<ISP_RAW_RAW10>(f, s, malvar, sharpen, cfg, st, ev, plain);
       break;
     default:
-      launch_main_f<ISP_RAW_RAW12>(f, s, malvar, sharpen, cfg, st, ev);
+      launch_main_f<ISP_RAW_RAW12>(f, s, malvar, sharpen, cfg, st, ev, plain);
       break;
   }
 }

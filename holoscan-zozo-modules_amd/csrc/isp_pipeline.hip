@@ -107,8 +107,10 @@ isp_status isp_process(isp_pipeline* pipeline, const isp_raw_view* raw, const is
     if (timing) pipeline->mark(s);
     // The frame-ordering event completes with the main kernel's dispatch
     // (cuda_pipeline.cu:359 records it after the last kernel instead).
+    const bool plain = f.histogram == nullptr && f.stats == nullptr && !pipeline->ae_enabled &&
+                       p.white_balance != ISP_WB_GRAY_WORLD;
     launch_main(f, pipeline->d_scratch, p.demosaic == ISP_DEMOSAIC_MALVAR, p.sharpen > 0.0F, pipeline->launch, s,
-                pipeline->done);
+                pipeline->done, plain);
     if (timing) pipeline->mark(s);
     pipeline->queued = true;
     pipeline->last_stream = s;

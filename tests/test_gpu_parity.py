@@ -86,6 +86,61 @@ def test_tiny_sizes(torch_gpu, size, pattern, method):
     compare(f"{w}x{h}", g, c)
 
 
+PLAIN_SIZES = [(16, 16), (33, 17), (70, 38), (257, 66), (1000, 17), (96, 130), (130, 47)]
+PLAIN_TINY = [(2, 2), (5, 3), (131, 3), (3, 37), (66, 2)]
+
+
+def plain_params(params, method, wb, k):
+    params.demosaic, params.sharpen, params.white_balance = method, 0.0, wb
+    if k % 2:
+        params.lens_shading = None
+    return params
+
+
+@pytest.mark.parametrize("fi", range(len(cases.FORMATS)))
+@pytest.mark.parametrize("method", [_abi.BILINEAR, _abi.MALVAR])
+@pytest.mark.parametrize("wb", [_abi.WB_MANUAL, _abi.WB_WHITE_PATCH])
+def test_plain_frames(torch_gpu, fi, method, wb):
+    """Frames with nothing to finish (no histograms or statistics, manual or white-patch
+    white balance, no sharpening) run the plain kernel (16-row tiles, 256-thread blocks);
+    every sample format, the patterns in turn, lens shading on and off, odd sizes, two
+    frames each (white patch smooths across them), then a frame with statistics on the
+    same pipeline (the full kernel after the plain one)."""
+    for k, (w, h) in enumerate(PLAIN_SIZES):
+        pattern = cases.PATTERNS[(fi + k) % len(cases.PATTERNS)]
+        config = cases.camera(pattern, cases.FORMATS[fi], w, h)
+        frame = isp.render_synthetic(config)
+        params = plain_params(isp.synthetic_params(config, 5), method, wb, k)
+        params.tone_curve = [_abi.TONE_SRGB, _abi.TONE_GAMMA, _abi.TONE_LINEAR][(fi + k) % 3]
+        out_format = (fi + k + method) % 4
+        gpu, cpu = isp.IspPipeline(params), OraclePipeline(params)
+        for with_stats in (False, False, True):
+            compare(f"{w}x{h} stats {with_stats}", gs.run(gpu, frame, out_format, with_stats, EXPOSURE),
+                    cpu.process(frame, out_format, with_stats, EXPOSURE))
+
+
+@pytest.mark.parametrize("fmt", [_abi.RAW_U8, _abi.RAW_U16])
+@pytest.mark.parametrize("method", [_abi.BILINEAR, _abi.MALVAR])
+def test_plain_tiny_frames(torch_gpu, fmt, method):
+    """The plain kernel on frames smaller than a tile in either direction (random samples, every
+    pattern in turn, white patch on every other size)."""
+    rng = np.random.default_rng(77 + fmt + 2 * method)
+    for k, (w, h) in enumerate(PLAIN_TINY):
+        pattern = cases.PATTERNS[k % len(cases.PATTERNS)]
+        if fmt == _abi.RAW_U8:
+            data = rng.integers(0, 256, size=w * h, dtype=np.uint8)
+            frame = isp.RawFrame(data, w, h, w, fmt, 8, pattern)
+        else:
+            img = rng.integers(0, 4096, size=(h, w), dtype=np.uint16)
+            frame = isp.RawFrame(img.view(np.uint8).ravel(), w, h, 2 * w, fmt, 12, pattern)
+        wb = _abi.WB_WHITE_PATCH if k % 2 else _abi.WB_MANUAL
+        params = isp.IspParams(black_level=16.0, wb_gains=(1.5, 1.0, 1.25), demosaic=method, white_balance=wb)
+        out_format = (k + method) % 4
+        gpu, cpu = isp.IspPipeline(params), OraclePipeline(params)
+        for _ in range(2):
+            compare(f"{w}x{h}", gs.run(gpu, frame, out_format, False), cpu.process(frame, out_format, False))
+
+
 @pytest.mark.parametrize("case", cases.float_edge_cases(), ids=lambda c: c[0])
 def test_float_edges(torch_gpu, case):
     """Subnormal samples (tiny white balance gains) and signed zeros reaching an RGB32F output,
@@ -94,6 +149,10 @@ def test_float_edges(torch_gpu, case):
     gpu, cpu = isp.IspPipeline(params), OraclePipeline(params)
     for _ in range(2):
         compare(name, gs.run(gpu, frame, out_format, True, EXPOSURE), cpu.process(frame, out_format, True, EXPOSURE))
+    # Without statistics (the plain kernel, where nothing else needs finishing).
+    gpu, cpu = isp.IspPipeline(params), OraclePipeline(params)
+    for _ in range(2):
+        compare(name + " plain", gs.run(gpu, frame, out_format, False), cpu.process(frame, out_format, False))
 
 
 @pytest.mark.parametrize("row", cases.LARGE_GRIDS, ids=lambda r: f"{r[0]}x{r[1]}x{r[2]} on {r[3]}x{r[4]}")
