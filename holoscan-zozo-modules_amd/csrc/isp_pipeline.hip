@@ -105,10 +105,12 @@ isp_status isp_process(isp_pipeline* pipeline, const isp_raw_view* raw, const is
     if (timing && awb) pipeline->mark(s);
     if (awb) launch_awb(f, pipeline->d_scratch, pipeline->launch, s);
     if (timing) pipeline->mark(s);
-    // The frame-ordering event completes with the main kernel's dispatch
-    // (cuda_pipeline.cu:359 records it after the last kernel instead).
+    // Nothing to finish after the tiles: the main pass can do without the
+    // histograms' LDS (more blocks a CU).
     const bool plain = f.histogram == nullptr && f.stats == nullptr && !pipeline->ae_enabled &&
                        p.white_balance != ISP_WB_GRAY_WORLD;
+    // The frame-ordering event completes with the main kernel's dispatch
+    // (cuda_pipeline.cu:359 records it after the last kernel instead).
     launch_main(f, pipeline->d_scratch, p.demosaic == ISP_DEMOSAIC_MALVAR, p.sharpen > 0.0F, pipeline->launch, s,
                 pipeline->done, plain);
     if (timing) pipeline->mark(s);
