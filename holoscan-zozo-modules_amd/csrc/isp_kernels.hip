@@ -1299,11 +1299,11 @@ __device__ __forceinline__ XcdShare xcd_share(int total) {
 // whose tile LDS (52 KB) lets three blocks in and whose bodies fit 80 VGPRs
 // without spilling (the packed formats' unpacking would spill). Three blocks
 // made the 1080p plain frame 8 % faster and the 4K one 7-9 %. The plain kernel
-// (256-thread blocks of 16-row tiles, no histograms) fits 80 VGPRs in every
-// format: six blocks a CU.
-template <int kFmt, bool kSharpen, bool kPlain = false>
+// (256-thread blocks of 16-row tiles, no histograms) runs at kPlainWaves: six
+// (80 VGPRs, every format without spilling) or eight (64 VGPRs, all but RAW10).
+template <int kFmt, bool kSharpen, bool kPlain = false, int kPlainWaves = 6>
 constexpr int main_waves_per_eu() {
-  return kPlain || (!kSharpen && (kFmt == ISP_RAW_U8 || kFmt == ISP_RAW_U16)) ? 6 : 4;
+  return kPlain ? kPlainWaves : !kSharpen && (kFmt == ISP_RAW_U8 || kFmt == ISP_RAW_U16) ? 6 : 4;
 }
 
 // One kTileH x 64 output tile; statistics sums into acc[0..2], saturated
@@ -1754,9 +1754,11 @@ __device__ __forceinline__ void main_finish(const FrameDesc& f, MainScratch* m, 
 // state or gray-world state; the host decides), in 256-thread blocks of 16-row
 // tiles without the histograms' LDS: more, smaller blocks a CU, so that a small
 // frame's tiles spread over the chip in fewer rounds.
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock, bool kZeroSign, bool kPlain = false>
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock, bool kZeroSign, bool kPlain = false,
+          int kPlainWaves = 6>
 __global__ void __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(kBlock >= 512 || kPlain ? main_waves_per_eu<kFmt, kSharpen, kPlain>() : 1)))
+__attribute__((amdgpu_waves_per_eu(kBlock >= 512 || kPlain ? main_waves_per_eu<kFmt, kSharpen, kPlain, kPlainWaves>()
+                                                           : 1)))
 main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows, int out_aligned) {
   using FL = std::conditional_t<kPlain, MainPlainFrameLds, MainFrameLds>;
   __shared__ FL fl;
@@ -1788,7 +1790,8 @@ main_kernel(FrameDesc f, DeviceScratch* s, int tiles_x, int tiles, int fast_rows
   // that neighbouring tiles, whose halos overlap, are read through one L2.
   const XcdShare share = xcd_share(tiles);
   for (int tile = share.first; tile < share.end; tile += share.stride) {
-    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign, main_waves_per_eu<kFmt, kSharpen, kPlain>() <= 4>(
+    main_tile<kFmt, kMalvar, kSharpen, kTileH, kBlock, kZeroSign,
+              main_waves_per_eu<kFmt, kSharpen, kPlain, kPlainWaves>() <= 4>(
         f, c, fl, tl, tile, tiles_x, fast_rows, out_aligned, acc);
   }
   // A frame with no histograms, statistics, exposure state or gray-world state
@@ -2193,11 +2196,11 @@ constexpr int kMainTileH = 32;
 template <bool kSharpen>
 constexpr int kMainBlock = 512;
 
-template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock, bool kPlain>
+template <int kFmt, bool kMalvar, bool kSharpen, int kTileH, int kBlock, bool kPlain, int kPlainWaves = 6>
 void launch_main_k(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream,
                    hipEvent_t done) {
-  auto kernel = f.zero_sign != 0 ? main_kernel<kFmt, kMalvar, kSharpen, kTileH, kBlock, true, kPlain>
-                                  : main_kernel<kFmt, kMalvar, kSharpen, kTileH, kBlock, false, kPlain>;
+  auto kernel = f.zero_sign != 0 ? main_kernel<kFmt, kMalvar, kSharpen, kTileH, kBlock, true, kPlain, kPlainWaves>
+                                  : main_kernel<kFmt, kMalvar, kSharpen, kTileH, kBlock, false, kPlain, kPlainWaves>;
   const int tiles_x = (f.width + 63) / 64;
   const int tiles_y = (f.height + kTileH - 1) / kTileH;
   const int tiles = tiles_x * tiles_y;
@@ -2215,13 +2218,27 @@ void launch_main_k(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg
 }
 
 // Frames with nothing to finish and no sharpening run the plain kernel: 16-row
-// tiles in 256-thread blocks without the histograms' LDS.
+// tiles in 256-thread blocks without the histograms' LDS. At six waves a SIMD a
+// tile's stages have more registers (a frame whose tiles all fit in one round of
+// the resident blocks runs about 15 % faster than at eight); at eight the chip
+// holds more blocks, which wins once the tiles outnumber the six-wave kernel's
+// (1080p: one round instead of two). RAW10's unpacking needs more registers
+// than eight waves leave.
 template <int kFmt, bool kMalvar, bool kSharpen>
 void launch_main_t(const FrameDesc& f, DeviceScratch* s, const LaunchConfig& cfg, hipStream_t stream,
                    hipEvent_t done, bool plain) {
   if constexpr (!kSharpen) {
     if (plain) {
-      launch_main_k<kFmt, kMalvar, false, 16, kThreads, true>(f, s, cfg, stream, done);
+      if constexpr (kFmt != ISP_RAW_RAW10) {
+        const int tiles = (f.width + 63) / 64 * ((f.height + 15) / 16);
+        const int resident = cfg.cu_count * blocks_per_cu(main_kernel<kFmt, kMalvar, false, 16, kThreads, false, true, 6>,
+                                                           kThreads);
+        if (tiles > resident) {
+          launch_main_k<kFmt, kMalvar, false, 16, kThreads, true, 8>(f, s, cfg, stream, done);
+          return;
+        }
+      }
+      launch_main_k<kFmt, kMalvar, false, 16, kThreads, true, 6>(f, s, cfg, stream, done);
       return;
     }
   }

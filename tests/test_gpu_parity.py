@@ -119,6 +119,23 @@ def test_plain_frames(torch_gpu, fi, method, wb):
                     cpu.process(frame, out_format, with_stats, EXPOSURE))
 
 
+@pytest.mark.parametrize("fi", range(len(cases.FORMATS)))
+def test_plain_large_frames(torch_gpu, fi):
+    """Plain frames with more 64 x 16 tiles than the six-wave plain kernel keeps resident
+    (2048 x 1200: 2,400 tiles), which run the eight-wave kernel (RAW10: the six-wave one),
+    both demosaics, lens shading on and off, white patch on the second."""
+    pattern = cases.PATTERNS[fi % len(cases.PATTERNS)]
+    config = cases.camera(pattern, cases.FORMATS[fi], 2048, 1200)
+    frame = isp.render_synthetic(config)
+    for k, method in enumerate((_abi.BILINEAR, _abi.MALVAR)):
+        params = plain_params(isp.synthetic_params(config, 5), method, (_abi.WB_MANUAL, _abi.WB_WHITE_PATCH)[k], k)
+        out_format = (fi + k) % 4
+        gpu, cpu = isp.IspPipeline(params), OraclePipeline(params)
+        for _ in range(2):
+            compare(f"2048x1200 {method}", gs.run(gpu, frame, out_format, False, EXPOSURE),
+                    cpu.process(frame, out_format, False, EXPOSURE, threads=8))
+
+
 @pytest.mark.parametrize("fmt", [_abi.RAW_U8, _abi.RAW_U16])
 @pytest.mark.parametrize("method", [_abi.BILINEAR, _abi.MALVAR])
 def test_plain_tiny_frames(torch_gpu, fmt, method):
