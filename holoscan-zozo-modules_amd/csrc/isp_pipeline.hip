@@ -110,9 +110,14 @@ isp_status isp_process(isp_pipeline* pipeline, const isp_raw_view* raw, const is
     const bool plain = f.histogram == nullptr && f.stats == nullptr && !pipeline->ae_enabled &&
                        p.white_balance != ISP_WB_GRAY_WORLD;
     // The frame-ordering event completes with the main kernel's dispatch
-    // (cuda_pipeline.cu:359 records it after the last kernel instead).
+    // (cuda_pipeline.cu:359 records it after the last kernel instead); a stream
+    // being captured into a graph gets the event record of its own, as there.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    check(hipStreamIsCapturing(s, &capture), "hipStreamIsCapturing");
+    const bool bind = capture == hipStreamCaptureStatusNone;
     launch_main(f, pipeline->d_scratch, p.demosaic == ISP_DEMOSAIC_MALVAR, p.sharpen > 0.0F, pipeline->launch, s,
-                pipeline->done, plain);
+                bind ? pipeline->done : nullptr, plain);
+    if (!bind) check(hipEventRecord(pipeline->done, s), "hipEventRecord");
     if (timing) pipeline->mark(s);
     pipeline->queued = true;
     pipeline->last_stream = s;
